@@ -1,0 +1,314 @@
+/*
+ * pxdock.h — C ABI of the MI355X (gfx950) docking hot path.
+ *
+ * Everything crossing this boundary is a plain C type: integers, doubles,
+ * pointers and sizes.  No exceptions cross it: every entry point returns a
+ * pxd_status and pxd_last_error() gives the text of the last failure on the
+ * calling thread.  Objects are opaque handles with an explicit destroy.
+ *
+ * The same descriptor structs (pxd_mol, pxd_frag_tree, pxd_leaves, pxd_box,
+ * pxd_score_params, pxd_lbfgs_params) are consumed by the FP64 CPU oracle that
+ * the tests use as the checker, so the GPU and the oracle read identical
+ * inputs.
+ *
+ * Reference interfaces replaced (paths relative to the Protenix-Dock tree):
+ *   receptor_cache::populate / get         engine/include/bytedock/core/grid.h:66-74
+ *   binding_system_interactions::put_gradients
+ *                                          engine/include/bytedock/core/interaction.h:105-108
+ *   optimization_step::apply (lbfgs_step)  engine/include/bytedock/core/optimizer.h:26-67
+ *   monte_carlo_searcher::fill             engine/include/bytedock/core/searcher.h:43
+ *   root_scorer::combine (pose_selection)  engine/include/bytedock/core/scorer.h:96-117
+ *   pose_cluster::preprocess               engine/include/bytedock/core/writer.h:53-67
+ */
+#ifndef PXDOCK_H_
+#define PXDOCK_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXD_NUM_VDW_TYPES 99 /* engine/include/bytedock/core/constant.h:27 */
+#define PXD_VDW_ID_OFFSET 1000 /* engine/include/bytedock/core/constant.h:28 */
+#define PXD_DIH_TERMS 6 /* engine/include/bytedock/core/system.h:48 */
+#define PXD_MAX_LIGAND_ATOMS 128 /* per-pose LDS budget of the batched kernels */
+#define PXD_MAX_FRAGMENTS 32 /* root + up to 31 rotatable bonds per ligand */
+#define PXD_MAX_LEAVES 64 /* receptor rotatable leaves */
+
+typedef enum pxd_status {
+  PXD_OK = 0,
+  PXD_ERR_ARG = 1,   /* invalid argument, shape or size */
+  PXD_ERR_CONF = 2,  /* malformed input data (failed_conf_error in the reference) */
+  PXD_ERR_HIP = 3,   /* HIP runtime error */
+  PXD_ERR_STATE = 4, /* call order misuse (e.g. no maps built) */
+  PXD_ERR_RANGE = 5  /* a size exceeds a documented maximum */
+} pxd_status;
+
+/* Force-field and interaction-set data of one molecule.
+ * Mirrors force_field_params (engine/include/bytedock/core/system.h:85-109).
+ * All index arrays are 0-based atom indices of this molecule.
+ * Ligand: vdw_type set (GAFF2 index 0..98, the JSON id minus 1000), sigma/epsilon NULL.
+ * Receptor: sigma/epsilon set, vdw_type NULL. */
+typedef struct pxd_mol {
+  int32_t n_atoms;
+  const int32_t* vdw_type;
+  const double* sigma;
+  const double* epsilon;
+  const double* charge;
+  /* harmonic bonds: ids[n][2], k, length (interaction.cpp:27-47) */
+  int32_t n_bonds;
+  const int32_t* bond_ij;
+  const double* bond_k;
+  const double* bond_r0;
+  /* harmonic angles: ids[n][3], k, angle in radians (interaction.cpp:49-74) */
+  int32_t n_angles;
+  const int32_t* angle_ijk;
+  const double* angle_k;
+  const double* angle_t0;
+  /* Fourier dihedrals: impropers first, then propers (interaction.cpp:104-117).
+   * ids[n][4]; periodicity[n][6] (a 0 ends the series), phase[n][6] radians, k[n][6] */
+  int32_t n_dihedrals;
+  const int32_t* dih_ijkl;
+  const int32_t* dih_n;
+  const double* dih_phase;
+  const double* dih_k;
+  /* non-bonded pairs: 1-4 (Coulomb x0.833333333, LJ x0.5) and all others (x1) */
+  int32_t n_pairs14;
+  const int32_t* pairs14;
+  int32_t n_pairs;
+  const int32_t* pairs;
+  /* pscore interaction sets (data.cpp:260-272) */
+  int32_t n_hydrophobic;
+  const int32_t* hydrophobic;
+  int32_t n_cation;
+  const int32_t* cation;
+  int32_t n_anion;
+  const int32_t* anion;
+  int32_t n_ring5;
+  const int32_t* ring5; /* [n][5] */
+  int32_t n_ring6;
+  const int32_t* ring6; /* [n][6] */
+  int32_t n_don_charged;
+  const int32_t* don_charged; /* [n][2] = (heavy donor, hydrogen) */
+  int32_t n_don_neutral;
+  const int32_t* don_neutral;
+  int32_t n_acc_charged;
+  const int32_t* acc_charged;
+  int32_t n_acc_neutral;
+  const int32_t* acc_neutral;
+  /* torsion-strain penalties (ligand only, data.cpp:204-223):
+   * coefs[n][4]; matches of penalty p are ijkl[match_off[p] .. match_off[p+1]) */
+  int32_t n_tstrain;
+  const double* tstrain_coef;
+  const int32_t* tstrain_match_off;
+  const int32_t* tstrain_ijkl;
+  int32_t n_rotatable;
+} pxd_mol;
+
+/* Rigid-fragment tree of a ligand (data.cpp:410-482), in a flat form.
+ * Atom positions are permuted so every fragment is a contiguous range.
+ * Torsion t (the order of the torsion vector in free_ligand::apply_parameters,
+ * data.h:78-81) rotates fragment tors_frag[t]; tors_frag lists the non-root
+ * fragments in the reference's pre-order traversal. */
+typedef struct pxd_frag_tree {
+  int32_t n_frags;
+  const int32_t* perm;        /* [n_atoms]: permuted slot p holds original atom perm[p] */
+  const int32_t* frag_start;  /* [n_frags] range [start, end) in permuted slots */
+  const int32_t* frag_end;
+  const int32_t* frag_parent; /* [n_frags] parent fragment, -1 for a root */
+  const int32_t* frag_bond;   /* [n_frags][2] (parent atom, mine atom) as permuted slots */
+  int32_t n_torsions;         /* = n_frags - number of roots */
+  const int32_t* tors_frag;   /* [n_torsions] */
+} pxd_frag_tree;
+
+/* Rotatable receptor leaves (data.cpp:348-363): leaf k rotates its movable
+ * atoms about origin[k] along the unit vector axis[k]. */
+typedef struct pxd_leaves {
+  int32_t n_leaves;
+  const double* origin;     /* [n][3] */
+  const double* axis;       /* [n][3] unit vectors */
+  const int32_t* atom_off;  /* [n+1] */
+  const int32_t* atoms;     /* movable receptor atoms */
+} pxd_leaves;
+
+/* Grid geometry (grid.h:31-40). */
+typedef struct pxd_box {
+  double init_xyz[3];
+  int32_t npoints[3];
+  double spacing;
+  double oppo_xyz[3];
+  int32_t npoints_minus_one[3];
+  double spacing_inv;
+} pxd_box;
+
+/* pose_selection parameters (conf.h:137-170).  A term is active when
+ * |coef| > 1e-5 (utility.h:24-26).  Inter-parameter sextets are
+ * (dist_min, dist_max, angle_min, angle_max, dist_center|sigma_min,
+ *  angle_center|sigma_max) in the order named per field below. */
+typedef struct pxd_score_params {
+  double coul_coef, vdw_coef, ionic_coef, pication_coef, f2f_coef, e2f_coef;
+  double hbond_coef, hydrophobic_coef, tstrain_coef;
+  /* interaction_vdw_energy */
+  double vdw_scale, vdw_cutoff;
+  double vdw_sigma_scale[PXD_NUM_VDW_TYPES];
+  double vdw_epsilon_scale[PXD_NUM_VDW_TYPES];
+  /* ionic_energy: coefs (lc, pc); ranges [2] = (dist_min, dist_max) */
+  double ionic_lc_coef, ionic_pc_coef;
+  double ionic_lc[2], ionic_pc[2];
+  /* pi_cationic_energy: order lc5, lc6, pc5, pc6 */
+  double pication_sub_coef[4];
+  double pication_range[4][2];
+  /* f2f / e2f pi stacking: order l5r5, l5r6, l6r5, l6r6;
+   * sextet = (dist_min, dist_max, angle_min, angle_max, sigma_min, sigma_max) */
+  double f2f_sub_coef[4];
+  double f2f_inter[4][6];
+  double e2f_sub_coef[4];
+  double e2f_inter[4][6];
+  /* hbond: scale; order hldcc, hldcn, hldnc, hldnn, hpdcc, hpdcn, hpdnc, hpdnn;
+   * sextet = (dist_min, dist_max, angle_min, angle_max, dist_center, angle_center) */
+  double hbond_scale;
+  double hbond_sub_coef[8];
+  double hbond_inter[8][6];
+  /* hydrophobic_energy */
+  double hydro_scale, hydro_dist_min, hydro_dist_max, hydro_scale_factor;
+} pxd_score_params;
+
+/* Number of pscore leaf terms reported per pose, in this order. */
+enum {
+  PXD_TERM_COULOMB = 0,
+  PXD_TERM_VDW,
+  PXD_TERM_IONIC,
+  PXD_TERM_PICATION,
+  PXD_TERM_F2F,
+  PXD_TERM_E2F,
+  PXD_TERM_HBOND,
+  PXD_TERM_HYDROPHOBIC,
+  PXD_TERM_TSTRAIN,
+  PXD_NUM_TERMS
+};
+
+/* L-BFGS with strong-Wolfe line search (optimizer.h:50-67, optimizer.cpp:235-560). */
+typedef struct pxd_lbfgs_params {
+  int32_t history;     /* reference: 100 */
+  int32_t max_iter;    /* reference: 240 (full) / relax_nsteps (relax) */
+  int32_t max_eval;    /* reference: max_iter * 5 / 4 */
+  double lr;           /* 1 */
+  double tol_grad;     /* infinity norm, 0.5 */
+  double tol_change;   /* 1e-4 */
+  double c1, c2;       /* 1e-4, 0.9 */
+  double ls_tol_change; /* 1e-9 */
+  double gtd_threshold; /* 1e5 */
+  int32_t max_ls;      /* 25 */
+} pxd_lbfgs_params;
+
+/* Fill the reference defaults (optimizer.h:52-66) for the given iteration cap. */
+void pxd_lbfgs_default_params(pxd_lbfgs_params* p, int32_t max_iter);
+
+/* ------------------------------------------------------------------------ */
+/* Library lifecycle and errors                                              */
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* pxd_last_error(void);
+/* Library version string. */
+const char* pxd_version(void);
+/* Number of visible HIP devices (0 when none), without creating a context. */
+pxd_status pxd_device_count(int32_t* out);
+
+/* Grid geometry helpers (grid.cpp:42-86). */
+void pxd_describe_boundary(const double center[3], const double size[3], pxd_box* out);
+void pxd_decide_grid_dims(const double center[3], const double size[3], double spacing,
+                          pxd_box* out);
+
+/* ------------------------------------------------------------------------ */
+/* Receptor (torsional_receptor, data.h:34-71)                               */
+
+typedef struct pxd_receptor pxd_receptor;
+
+/* Copies the receptor's coordinates, force field, interaction sets and leaves
+ * to device memory.  xyz is [n_atoms][3]. */
+pxd_status pxd_receptor_create(const pxd_mol* mol, const double* xyz,
+                               const pxd_leaves* leaves, pxd_receptor** out);
+void pxd_receptor_destroy(pxd_receptor* r);
+
+/* ------------------------------------------------------------------------ */
+/* Grid maps (receptor_cache, grid.h:60-133)                                 */
+
+typedef struct pxd_grid pxd_grid;
+
+/* Builds the Coulomb map and one LJ map per requested GAFF2 type on the GPU,
+ * each point holding (value, dE/dx, dE/dy, dE/dz) in FP32 (grid.cpp:115-195).
+ * Movable receptor atoms are excluded (grid.cpp:118-123).  slope is the
+ * out-of-box wall (grid.cpp:246-251; set_slope grid.h:79). */
+pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                          int32_t n_types, double slope, void* stream, pxd_grid** out);
+/* Wraps maps computed elsewhere (e.g. loaded from disk).  maps is host FP32
+ * [(n_types+1)][npoints][4], map 0 = Coulomb. */
+pxd_status pxd_grid_upload(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                           int32_t n_types, double slope, const float* maps, pxd_grid** out);
+/* Copies the maps back to host FP32 in the upload layout. */
+pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats);
+pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types);
+pxd_status pxd_grid_set_slope(pxd_grid* g, double slope);
+void pxd_grid_destroy(pxd_grid* g);
+
+/* ------------------------------------------------------------------------ */
+/* Ligand batch (free_ligand, data.h:73-99) packed for uneven sizes          */
+
+typedef struct pxd_ligand_batch pxd_ligand_batch;
+
+/* Packs n ligands (term lists, types, charges, interaction sets and fragment
+ * trees; trees may be NULL) into device CSR arrays.  Every ligand must use a
+ * type present in the grid it is scored against. */
+pxd_status pxd_ligand_batch_create(const pxd_mol* ligands, const pxd_frag_tree* trees,
+                                   int32_t n, pxd_ligand_batch** out);
+void pxd_ligand_batch_destroy(pxd_ligand_batch* b);
+/* atom count of ligand i */
+pxd_status pxd_ligand_batch_natoms(const pxd_ligand_batch* b, int32_t i, int32_t* out);
+
+/* ------------------------------------------------------------------------ */
+/* Poses. A pose set is CSR: pose p is ligand pose_lig[p], its coordinates    */
+/* are xyz[3*pose_atom_off[p] ...] (FP32, device memory), its receptor       */
+/* torsions tors[p*n_leaves ...].  All pose arrays below are DEVICE pointers. */
+
+/* Batched energy + analytic gradient (binding_system_interactions::put_gradients,
+ * interaction.cpp:271-298) for n_poses poses, one wavefront per pose.
+ * grid may be NULL: then the inter term is the brute-force sum over all
+ * receptor atoms (interaction.cpp:241-254).  Outputs are OVERWRITTEN:
+ * energy[p] (FP64), lig_grad (FP32, same CSR as xyz), tors_grad[p*n_leaves+k] (FP32). */
+pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* grid,
+                                const pxd_ligand_batch* b, int32_t n_poses,
+                                const int32_t* pose_lig, const int32_t* pose_atom_off,
+                                const float* xyz, const float* tors, double* energy,
+                                float* lig_grad, float* tors_grad, void* stream);
+
+/* Batched persistent L-BFGS (lbfgs_step::apply, optimizer.cpp:429-560):
+ * xyz and tors are optimised IN PLACE.  Outputs per pose: energy (final
+ * evaluation), nevals, converged (1/0).  Many uneven instances run in one
+ * launch from a device work queue.  workspace may be NULL (the library then
+ * allocates history storage for its slots). */
+pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* grid,
+                           const pxd_ligand_batch* b, const pxd_lbfgs_params* prm,
+                           int32_t n_poses, const int32_t* pose_lig,
+                           const int32_t* pose_atom_off, float* xyz, float* tors,
+                           double* energy, int32_t* nevals, int32_t* converged,
+                           void* stream);
+
+/* Batched pscore (root_scorer::combine on the pose_selection tree,
+ * scorer.cpp:117-205 and leaves :207-613) for n_poses poses.  Receptor
+ * coordinates come from applying tors to the receptor's leaves
+ * (data.cpp:366-385); tors may be NULL (all zero).  score[p] (FP64) is the
+ * weighted sum; terms (may be NULL) receives [n_poses][PXD_NUM_TERMS] unweighted
+ * leaf values (FP64). */
+pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
+                            const pxd_score_params* sp, int32_t n_poses,
+                            const int32_t* pose_lig, const int32_t* pose_atom_off,
+                            const float* xyz, const float* tors, double* score,
+                            double* terms, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PXDOCK_H_ */

@@ -1,0 +1,137 @@
+/*
+ * pxd_oracle.h — FP64 CPU restatement of the Protenix-Dock docking hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product (libpxdock.so) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the Protenix-Dock tree).  Inputs are the descriptor structs of
+ * include/pxdock.h, so the oracle and the GPU read the same data.
+ */
+#ifndef PXD_ORACLE_H_
+#define PXD_ORACLE_H_
+
+#include <stdint.h>
+
+#include "pxdock.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants (system.h:83, math.h:128, constant.h:29-130) ---- */
+extern const double or_coulomb_factor;
+extern const double or_gaff2_sigma[PXD_NUM_VDW_TYPES];
+extern const double or_gaff2_epsilon[PXD_NUM_VDW_TYPES];
+
+/* ---- lib/math (math.h, math.cpp) ---- */
+void or_matmul3(const double* a, const double* b, double* out);
+void or_rotmat_quat(const double q[4], double out[9]);
+void or_rotmat_axis(const double axis[3], double theta, double out[9]);
+void or_rotmat_axis_grad(const double axis[3], double theta, double out[9], double jac[9]);
+double or_angle_grad(const double* pi, const double* pj, const double* pk, double* gi,
+                     double* gk);
+double or_dihedral(const double* pi, const double* pj, const double* pk, const double* pl);
+double or_dihedral_grad(const double* pi, const double* pj, const double* pk,
+                        const double* pl, double* gi, double* gj, double* gk, double* gl);
+double or_rev_linear(double v, double minv, double maxv);
+double or_rev_linear_c(double v, double center, double minv, double maxv);
+
+/* ---- pair terms (pair.h:25-60) ---- */
+double or_coul_pair(double qi, double qj, const double* rij, double d, double scale,
+                    double* gi, double* gj);
+double or_vdw_pair(double si, double ei, double sj, double ej, const double* rij, double d,
+                   double scale, double* gi, double* gj);
+
+/* ---- intra-molecular terms (interaction.cpp:27-196) ---- */
+double or_bond_energy(const pxd_mol* m, const double* xyz, double* grad);
+double or_angle_energy(const pxd_mol* m, const double* xyz, double* grad);
+double or_dihedral_energy(const pxd_mol* m, const double* xyz, double* grad);
+double or_nonbonded_energy(const pxd_mol* m, const double* xyz, double* grad);
+double or_intra_energy(const pxd_mol* m, const double* xyz, double* grad);
+
+/* ---- grid maps (grid.cpp:115-303) ---- */
+typedef struct or_grid {
+  pxd_box box;
+  int32_t n_types;
+  const int32_t* types;   /* GAFF2 index of map t+1 */
+  const double* maps;     /* [(n_types+1)][P][4], map 0 = Coulomb */
+  double slope;
+  int32_t n_excluded;     /* movable receptor atoms, computed brute force */
+  const int32_t* excluded;
+} or_grid;
+
+/* Values of the Coulomb map and the requested type maps at one point:
+ * out[(n_types+1)*4] (grid.cpp:164-194 for a single point). */
+void or_grid_point(const pxd_mol* rec, const double* rec_xyz, const uint8_t* excluded,
+                   const double* p, const int32_t* types, int32_t n_types, double* out);
+/* Full build, threaded over x-slabs (grid.cpp:115-148). */
+void or_grid_build(const pxd_mol* rec, const double* rec_xyz, const uint8_t* excluded,
+                   const pxd_box* box, const int32_t* types, int32_t n_types, double* maps,
+                   int32_t nthreads);
+/* receptor_cache::get (grid.cpp:197-262): adds into energy/grad, returns hit. */
+int32_t or_grid_get(const or_grid* g, int32_t type, double q, const double* xyz,
+                    double* energy, double* grad);
+
+/* ---- inter + total energy (interaction.cpp:202-350) ---- */
+double or_inter_energy(const pxd_mol* rec, const double* rec_xyz, const or_grid* grid,
+                       const pxd_mol* lig, const double* lig_xyz, double* rec_grad,
+                       double* lig_grad);
+
+typedef struct or_system {
+  const pxd_mol* rec;
+  const double* rec_xyz;     /* initial receptor coordinates */
+  const pxd_leaves* leaves;
+  const pxd_mol* lig;
+  const or_grid* grid;       /* NULL: brute-force inter */
+} or_system;
+
+/* binding_system_interactions::put_gradients (interaction.cpp:271-298).
+ * Gradients are OVERWRITTEN (the caller-side reset of interaction.cpp:309-314
+ * is folded in).  tors/tors_grad have leaves->n_leaves entries. */
+double or_system_energy(const or_system* s, const double* tors, const double* lig_xyz,
+                        double* tors_grad, double* lig_grad);
+
+/* ---- L-BFGS (optimizer.cpp:205-560) ---- */
+/* Optimises (lig_xyz, tors) in place; returns converged; energy/nevals out. */
+int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lig_xyz,
+                 double* tors, double* energy, int32_t* nevals);
+
+/* Threaded batch forms for the CPU baseline: systems[] one per ligand. */
+void or_score_many(const or_system* systems, int32_t n_poses, const int32_t* pose_lig,
+                   const int32_t* pose_atom_off, const double* xyz, const double* tors,
+                   int32_t n_leaves, double* energy, double* lig_grad, double* tors_grad,
+                   int32_t nthreads);
+void or_lbfgs_many(const or_system* systems, const pxd_lbfgs_params* p, int32_t n_poses,
+                   const int32_t* pose_lig, const int32_t* pose_atom_off, double* xyz,
+                   double* tors, int32_t n_leaves, double* energy, int32_t* nevals,
+                   int32_t* converged, int32_t nthreads);
+
+/* ---- conformations (data.cpp:366-385, 522-641) ---- */
+void or_receptor_apply(const pxd_leaves* L, int32_t n_atoms, const double* xyz,
+                       const double* tors, double* out);
+void or_ligand_apply(const pxd_frag_tree* t, int32_t n_atoms, const double* xyz,
+                     const double center[3], const double rot[9], const double* tors,
+                     double* out);
+
+/* ---- pscore (scorer.cpp:117-613) ---- */
+double or_pscore(const pxd_score_params* sp, const pxd_mol* rec, const double* rec_xyz,
+                 const pxd_mol* lig, const double* lig_xyz, double* terms);
+/* the vdW leaf alone, as used by randomize (searcher.cpp:75-77) */
+double or_pscore_vdw(const pxd_score_params* sp, const pxd_mol* rec, const double* rec_xyz,
+                     const pxd_mol* lig, const double* lig_xyz);
+
+/* ---- counter-based RNG shared with the GPU (Philox4x32-10) ---- */
+void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- top-k clustering (writer.cpp:99-177) ---- */
+int32_t or_pose_cluster(int32_t n_cand, int32_t n_atoms, const double* xyz,
+                        const double* pscore, const int32_t* nevals, int32_t num_modes,
+                        double min_rmsd, int32_t* out_index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PXD_ORACLE_H_ */
