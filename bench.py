@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""Benchmark of the docking hot path on one MI355X.
+
+A step is one batched score+gradient pass (binding_system_interactions::
+put_gradients for every pose) over the batched-score workload: 4,096
+synthetic ligands (10-60 heavy atoms, 0-15 torsions, 0-3 rings) x 64 random
+poses = 262,144 evaluations against a 2,000-heavy-atom receptor with 8
+rotatable hydroxyls and a 61^3 grid (22.5 A box at 0.375 A).  Inputs are
+resident in HBM before the timed region.  The JSON line also carries
+L-BFGS-converged poses/s of the persistent batched kernel vs the plain
+one-pose-per-launch form, the CPU baseline, and the roofline of the score
+kernel.  Diagnostics go to stderr; stdout holds exactly one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "protenix-dock_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _make_ligand(i):
+    from pxdock_amd import synth
+    rng = np.random.default_rng(10000 + i + 7919)
+    nh = int(rng.integers(10, 61))
+    nt = int(rng.integers(0, 16))
+    nr = int(rng.integers(0, 4))
+    return synth.make_ligand(10000 + i, n_heavy=nh, n_torsions=nt, n_rings=nr)
+
+
+def _make_poses(args):
+    from pxdock_amd import synth
+    i, lig, lo, hi, n = args
+    rng = np.random.default_rng(20000 + i)
+    out = np.empty((n, lig.n_atoms, 3), np.float32)
+    for k in range(n):
+        c = rng.uniform(lo, hi)
+        R = synth.quat_to_rot(synth.random_quaternion(rng))
+        t = rng.uniform(-math.pi, math.pi, lig.tree.n_torsions)
+        out[k] = synth.ligand_apply(lig.tree, lig.xyz[0], c, R, t)
+    return out
+
+
+def workload(n_lig, per_lig, workers):
+    from concurrent.futures import ProcessPoolExecutor
+
+    from pxdock_amd import lib, synth
+    rec = synth.make_receptor(1)
+    box = lib.decide_grid_dims(rec.pocket_center, [22.5] * 3, 0.375)
+    lo, hi = np.array(box.init_xyz), np.array(box.oppo_xyz)
+    with ProcessPoolExecutor(workers) as ex:
+        ligs = list(ex.map(_make_ligand, range(n_lig), chunksize=16))
+        poses = list(ex.map(_make_poses, [(i, l, lo, hi, per_lig) for i, l in enumerate(ligs)],
+                            chunksize=16))
+    return rec, box, ligs, poses
+
+
+def flops_bytes_per_pose(lig, rec_pairs, n_mov, n_rec_bonded):
+    """Algorithmic work of one score+grad evaluation (counting: add/mul = 1,
+    FMA = 2, div/sqrt/transcendental = 1): grid gather 256 B per atom (2 maps x
+    8 corners x 4 FP32 channels); FLOPs as 160 per atom (lookup), 35 per
+    non-bonded pair, 25 per bond, 60 per angle, 150 per dihedral."""
+    n = lig.n_atoms
+    b = 256.0 * n
+    f = (160.0 * n + 35.0 * (len(lig.pairs) + len(lig.pairs14) + rec_pairs + n * n_mov)
+         + 25.0 * len(lig.bond_ij) + 60.0 * len(lig.angle_ijk) + 150.0 * len(lig.dih_ijkl)
+         + 100.0 * n_rec_bonded)
+    return f, b
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ligands", type=int, default=4096)
+    ap.add_argument("--poses-per-ligand", type=int, default=64)
+    ap.add_argument("--lbfgs-poses", type=int, default=32768)
+    ap.add_argument("--plain-poses", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lbfgs", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X; "
+              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from pxdock_amd import ffi, lib
+    workers = min(os.cpu_count() or 1, 16)
+    t0 = time.time()
+    rec, box, ligs, poses = workload(args.ligands, args.poses_per_ligand, workers)
+    log(f"workload generated in {time.time() - t0:.1f}s: {len(ligs)} ligands, "
+        f"mean atoms {np.mean([l.n_atoms for l in ligs]):.1f}")
+    types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
+    R = lib.Receptor(rec)
+    torch.cuda.synchronize()
+    tg0 = time.time()
+    G = lib.Grid(R, box, types)
+    torch.cuda.synchronize()
+    grid_ms = (time.time() - tg0) * 1e3
+    grid_bytes = (len(types) + 1) * box.n_points() * 16
+    B = lib.LigandBatch(ligs)
+    pose_lig = np.repeat(np.arange(len(ligs)), args.poses_per_ligand)
+    xyz_list = [p for arr in poses for p in arr]
+    P = lib.PoseSet(B, pose_lig, xyz_list, rec.leaves.n_leaves)
+    n_poses = P.n_poses
+    log(f"{n_poses} poses resident on the device; grid {box.npts} x {len(types) + 1} maps "
+        f"built in {grid_ms:.1f} ms")
+
+    # ---------------- timed region: K batched score+grad steps
+    for _ in range(args.warmup):
+        lib.score_grad(R, G, P)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(args.steps):
+        e, g, tg = lib.score_grad(R, G, P)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    value = n_poses / (ms / 1e3)
+    finite = float(torch.isfinite(e).float().mean().item())
+    log(f"score+grad: {ms:.3f} ms/step -> {value:.4e} evals/s (finite {finite:.4f})")
+
+    # ---------------- roofline of the score kernel
+    rec_pairs = len(rec.pairs) + len(rec.pairs14)
+    n_mov = len(rec.leaves.atoms)
+    nb = len(rec.bond_ij) + len(rec.angle_ijk) + len(rec.dih_ijkl)
+    F = Bt = 0.0
+    for l, lig in enumerate(ligs):
+        f, b = flops_bytes_per_pose(lig, rec_pairs, n_mov, nb)
+        F += f * args.poses_per_ligand
+        Bt += b * args.poses_per_ligand
+    t_s = ms / 1e3
+    t_hbm, t_flop = Bt / (HBM_PEAK_GBS * 1e9), F / (FP32_PEAK_TFLOPS * 1e12)
+    if t_flop >= t_hbm:
+        roof = {"bound": "mfma", "achieved": F / t_s / 1e12, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s"}
+    else:
+        roof = {"bound": "hbm", "achieved": Bt / t_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = "k_score"
+    roof["algorithmic_bytes_per_launch"] = Bt
+    roof["algorithmic_flops_per_launch"] = F
+
+    out = {
+        "metric": "L-BFGS-converged poses/sec + score+grad evals/sec vs plain GPU and CPU",
+        "value": value, "unit": "score+grad evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 energy reduction)",
+        "data": "synthetic (seeded receptor / ligands / poses, generated in-process)",
+        "config": {"workload": "batched score+grad: 4096 ligands (10-60 heavy atoms, 0-15 "
+                               "torsions, 0-3 rings) x 64 poses = 262144 evaluations; receptor "
+                               "2000 heavy atoms + polar H, 8 rotatable hydroxyls; grid 22.5 A "
+                               "@ 0.375 A (61^3), FP32 value+gradient maps",
+                   "ligands": len(ligs), "poses_per_ligand": args.poses_per_ligand,
+                   "poses": n_poses, "grid_points": box.npts, "maps": len(types) + 1},
+        "roofline": roof,
+        "grid_build": {"ms": grid_ms, "output_GB_per_s": grid_bytes / (grid_ms / 1e3) / 1e9,
+                       "maps": len(types) + 1, "points": box.n_points()},
+    }
+
+    # ---------------- L-BFGS: persistent batched vs plain one-pose-per-launch
+    if not args.no_lbfgs:
+        prm = ffi.lbfgs_params(240)  # reference full step: m=100, 240 iters, |g|inf <= 0.5
+        nl = min(args.lbfgs_poses, n_poses)
+        idx = np.linspace(0, n_poses - 1, nl).astype(int)
+        sub_xyz = [xyz_list[i] for i in idx]
+        Pl = lib.PoseSet(B, pose_lig[idx], sub_xyz, rec.leaves.n_leaves)
+        lib.lbfgs(R, G, lib.PoseSet(B, pose_lig[idx[:256]], sub_xyz[:256], rec.leaves.n_leaves), prm)
+        torch.cuda.synchronize()
+        t1 = time.time()
+        le, lne, lcv = lib.lbfgs(R, G, Pl, prm)
+        torch.cuda.synchronize()
+        dt = time.time() - t1
+        lcv = lcv.cpu().numpy()
+        lne = lne.cpu().numpy()
+        batched = {"poses": nl, "seconds": dt, "converged_frac": float(lcv.mean()),
+                   "converged_poses_per_s": float(lcv.sum() / dt),
+                   "evals_per_s": float(lne.sum() / dt)}
+        npl = min(args.plain_poses, nl)
+        Pp = lib.PoseSet(B, pose_lig[idx[:npl]], sub_xyz[:npl], rec.leaves.n_leaves)
+        torch.cuda.synchronize()
+        t2 = time.time()
+        pe, pne, pcv = lib.lbfgs_plain(R, G, Pp, prm)
+        torch.cuda.synchronize()
+        dtp = time.time() - t2
+        pcv = pcv.cpu().numpy()
+        plain = {"poses": npl, "seconds": dtp, "converged_frac": float(pcv.mean()),
+                 "converged_poses_per_s": float(max(pcv.sum(), 1) / dtp),
+                 "evals_per_s": float(pne.cpu().numpy().sum() / dtp)}
+        # same-start agreement of the two GPU forms (same kernel, same tolerance)
+        le_np = le.cpu().numpy()[:npl]
+        pe_np = pe.cpu().numpy()
+        agree = float(np.mean(np.abs(le_np - pe_np) <= np.maximum(1e-3, 1e-4 * np.abs(pe_np))))
+        out["lbfgs"] = {"params": "reference full step: history 100, 240 iterations, "
+                                  "|g|_inf <= 0.5, strong Wolfe",
+                        "batched": batched, "plain_gpu": plain,
+                        "speedup_converged_vs_plain": batched["converged_poses_per_s"] /
+                        plain["converged_poses_per_s"],
+                        "batched_vs_plain_same_energy_frac": agree}
+        log(f"L-BFGS batched {batched}; plain {plain}")
+
+    # ---------------- CPU baseline: the FP64 oracle on the host cores
+    if not args.no_cpu:
+        import pyoracle as O
+        maps = G.download().astype(np.float64)
+        og = O.Grid(box, G.types, maps, 1e6, rec.leaves.movable())
+        threads = min(os.cpu_count() or 1, 16)
+        systems = [O.System(rec, lig, og) for lig in ligs]
+        flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in xyz_list])
+        offs = P.pose_atom_off
+        # calibrate, then time a bounded sample of about --cpu-seconds
+        n0 = 512
+        t3 = time.time()
+        O.score_many(systems, pose_lig[:n0], offs[:n0], flat, None, rec.leaves.n_leaves, threads)
+        rate0 = n0 / (time.time() - t3)
+        ns = int(min(n_poses, max(n0, rate0 * args.cpu_seconds)))
+        sl = np.linspace(0, n_poses - 1, ns).astype(int)
+        t4 = time.time()
+        O.score_many(systems, pose_lig[sl].astype(np.int32), offs[sl].astype(np.int32), flat, None,
+                     rec.leaves.n_leaves, threads)
+        cpu_rate = ns / (time.time() - t4)
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "score+grad evals/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{ns} of the {n_poses} poses (evenly strided), FP64 "
+                                         f"oracle, {threads} threads, same maps"}
+        out["speedup_vs_cpu"] = value / cpu_rate
+        log(f"cpu baseline {cpu_rate:.4e} evals/s on {threads} threads ({ns} poses)")
+        if "lbfgs" in out:
+            nc = 64
+            prm = ffi.lbfgs_params(240)
+            t5 = time.time()
+            _, _, _, _, cv = O.lbfgs_many(systems, prm, pose_lig[idx[:nc]].astype(np.int32),
+                                          offs[idx[:nc]].astype(np.int32), flat,
+                                          np.zeros((n_poses, rec.leaves.n_leaves)),
+                                          rec.leaves.n_leaves, threads)
+            dtc = time.time() - t5
+            out["lbfgs"]["cpu"] = {"poses": nc, "seconds": dtc, "cores": threads,
+                                   "converged_poses_per_s": float(max(cv.sum(), 1) / dtc)}
+            out["lbfgs"]["speedup_converged_vs_cpu"] = (
+                out["lbfgs"]["batched"]["converged_poses_per_s"] /
+                out["lbfgs"]["cpu"]["converged_poses_per_s"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

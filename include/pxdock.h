@@ -295,6 +295,18 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* grid,
                            double* energy, int32_t* nevals, int32_t* converged,
                            void* stream);
 
+/* The plain GPU form of pxd_lbfgs_batch, kept as the comparison baseline:
+ * the optimiser runs on the host (FP64) and every energy evaluation is one
+ * launch of the same score+grad kernel for one pose, followed by a copy back
+ * and a stream synchronisation.  Same arguments and results as
+ * pxd_lbfgs_batch (lbfgs_step::apply, optimizer.cpp:429-560). */
+pxd_status pxd_lbfgs_plain(const pxd_receptor* r, const pxd_grid* grid,
+                           const pxd_ligand_batch* b, const pxd_lbfgs_params* prm,
+                           int32_t n_poses, const int32_t* pose_lig,
+                           const int32_t* pose_atom_off, float* xyz, float* tors,
+                           double* energy, int32_t* nevals, int32_t* converged,
+                           void* stream);
+
 /* Batched pscore (root_scorer::combine on the pose_selection tree,
  * scorer.cpp:117-205 and leaves :207-613) for n_poses poses.  Receptor
  * coordinates come from applying tors to the receptor's leaves

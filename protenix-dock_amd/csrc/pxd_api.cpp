@@ -1,0 +1,907 @@
+// pxd_api.cpp — the C ABI (include/pxdock.h): validation, packing of the
+// receptor / ligand descriptors into device arrays, and kernel launches.
+// No exception crosses the boundary; failures set a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pxd_internal.hpp"
+#include "pxd_kernels.hpp"
+#include "pxdock.h"
+#include "pxdock_vdw_table.h"
+
+using namespace pxd;
+
+namespace {
+
+thread_local std::string g_error;
+
+pxd_status fail(pxd_status st, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return st;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(PXD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
+  } while (0)
+
+#define SIGMA_ROW(i, tag, s, e) s,
+#define EPSILON_ROW(i, tag, s, e) e,
+const double kGaffSigma[PXD_NUM_VDW_TYPES] = {PXD_VDW_TABLE(SIGMA_ROW)};
+const double kGaffEpsilon[PXD_NUM_VDW_TYPES] = {PXD_VDW_TABLE(EPSILON_ROW)};
+
+// Host staging buffer that lays several typed arrays into one device allocation.
+struct Packer {
+  std::vector<char> host;
+  size_t add(const void* src, size_t bytes) {
+    size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes + 1);
+    if (bytes && src) memcpy(host.data() + off, src, bytes);
+    return off;
+  }
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    return add(v.data(), v.size() * sizeof(T));
+  }
+};
+
+template <class T>
+const T* dptr(void* base, size_t off) {
+  return reinterpret_cast<const T*>(static_cast<char*>(base) + off);
+}
+
+uint32_t pack_periods(const int32_t* per, int* bad) {
+  uint32_t v = 0;
+  for (int n = 0; n < PXD_DIH_TERMS; ++n) {
+    int p = per[n];
+    if (p < 0 || p > 15) *bad = 1;
+    if (p == 0) break;
+    v |= uint32_t(p & 15) << (4 * n);
+  }
+  return v;
+}
+
+void pack_kp(const double* k, const double* ph, const int32_t* per, float4* out) {
+  float kk[6], pp[6];
+  for (int n = 0; n < 6; ++n) {
+    kk[n] = (float)k[n];
+    pp[n] = (float)ph[n];
+  }
+  out[0] = make_float4(kk[0], kk[1], kk[2], kk[3]);
+  out[1] = make_float4(kk[4], kk[5], pp[0], pp[1]);
+  out[2] = make_float4(pp[2], pp[3], pp[4], pp[5]);
+  (void)per;
+}
+
+// ---- host FP64 energies of receptor terms that touch no movable atom: they
+// are pose-independent and enter every pose as one constant.
+double h_dist(const double* a, const double* b) {
+  double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+double h_dihedral(const double* pi, const double* pj, const double* pk, const double* pl) {
+  double rij[3], rkj[3], rkl[3], m[3], n[3], w[3];
+  for (int d = 0; d < 3; ++d) {
+    rij[d] = pj[d] - pi[d];
+    rkj[d] = pj[d] - pk[d];
+    rkl[d] = pl[d] - pk[d];
+  }
+  auto cr = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  cr(rij, rkj, m);
+  cr(rkj, rkl, n);
+  cr(m, n, w);
+  double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double phi = std::atan2(wl, m[0] * n[0] + m[1] * n[1] + m[2] * n[2] + 1e-20);
+  return (rij[0] * n[0] + rij[1] * n[1] + rij[2] * n[2]) < 0 ? phi : -phi;
+}
+
+double h_pair(double qq, double sig, double e4, double d) {
+  double u6 = std::pow(sig / (d + 1e-20), 6.0);
+  return qq / (d + 1e-20) * 332.0637141491396 + (u6 * u6 - u6) * e4;
+}
+
+struct LbfgsWorkspace {
+  std::mutex mu;
+  float* hist = nullptr;
+  size_t hist_floats = 0;
+  int* counter = nullptr;
+};
+LbfgsWorkspace g_ws;
+
+}  // namespace
+
+extern "C" {
+
+const char* pxd_last_error(void) { return g_error.c_str(); }
+
+// internal: lets the other translation units record an error
+pxd_status pxd_set_error_(pxd_status st, const char* msg) { return fail(st, "%s", msg); }
+
+const char* pxd_version(void) { return "pxdock-mi355x 0.1 (gfx950)"; }
+
+pxd_status pxd_device_count(int32_t* out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *out = n;
+  return PXD_OK;
+}
+
+void pxd_lbfgs_default_params(pxd_lbfgs_params* p, int32_t max_iter) {
+  p->history = 100;
+  p->max_iter = max_iter;
+  p->max_eval = max_iter * 5 / 4;
+  p->lr = 1.0;
+  p->tol_grad = 0.5;
+  p->tol_change = 1e-4;
+  p->c1 = 1e-4;
+  p->c2 = 0.9;
+  p->ls_tol_change = 1e-9;
+  p->gtd_threshold = 1e5;
+  p->max_ls = 25;
+}
+
+// grid.cpp:42-55
+void pxd_describe_boundary(const double c[3], const double s[3], pxd_box* b) {
+  memset(b, 0, sizeof(*b));
+  for (int d = 0; d < 3; ++d) {
+    double h = s[d] * 0.5;
+    b->init_xyz[d] = c[d] - h;
+    b->oppo_xyz[d] = c[d] + h;
+  }
+}
+
+// grid.cpp:57-86
+void pxd_decide_grid_dims(const double c[3], const double s[3], double spacing, pxd_box* b) {
+  memset(b, 0, sizeof(*b));
+  for (int d = 0; d < 3; ++d) {
+    long nv = (long)std::ceil(s[d] / spacing);
+    if (nv % 2 == 1) nv += 1;
+    double len = nv * spacing;
+    b->npoints[d] = (int32_t)(nv + 1);
+    b->init_xyz[d] = c[d] - len * 0.5;
+  }
+  b->spacing = spacing;
+  b->spacing_inv = 1.0 / spacing;
+  for (int d = 0; d < 3; ++d) {
+    b->npoints_minus_one[d] = b->npoints[d] - 1;
+    b->oppo_xyz[d] = b->init_xyz[d] + b->npoints_minus_one[d] * spacing;
+  }
+}
+
+// ------------------------------------------------------------------ receptor
+
+pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_leaves* lv,
+                               pxd_receptor** out) {
+  if (!m || !xyz || !out) return fail(PXD_ERR_ARG, "receptor: null argument");
+  *out = nullptr;
+  const int n = m->n_atoms;
+  if (n <= 0) return fail(PXD_ERR_ARG, "receptor: no atoms");
+  if (!m->sigma || !m->epsilon || !m->charge)
+    return fail(PXD_ERR_CONF, "receptor: sigma/epsilon/charge required");
+  const int nl = lv ? lv->n_leaves : 0;
+  if (nl > PXD_MAX_LEAVES) return fail(PXD_ERR_RANGE, "receptor: %d leaves > %d", nl, PXD_MAX_LEAVES);
+  // movable slots ordered by leaf (receptor_torsion2xyz order, interaction.cpp:323-335)
+  std::vector<int> slot_of(n, -1), mov_leaf, mov_atom;
+  for (int k = 0; k < nl; ++k)
+    for (int a = lv->atom_off[k]; a < lv->atom_off[k + 1]; ++a) {
+      int j = lv->atoms[a];
+      if (j < 0 || j >= n) return fail(PXD_ERR_CONF, "receptor: leaf atom %d out of range", j);
+      if (slot_of[j] >= 0)
+        return fail(PXD_ERR_CONF, "receptor: atom %d belongs to two leaves (unsupported)", j);
+      slot_of[j] = (int)mov_atom.size();
+      mov_leaf.push_back(k);
+      mov_atom.push_back(j);
+    }
+  const int M = (int)mov_atom.size();
+  if (M > kMaxMovable) return fail(PXD_ERR_RANGE, "receptor: %d movable atoms > %d", M, kMaxMovable);
+  auto ref = [&](int a) { return slot_of[a] >= 0 ? mov_ref(slot_of[a]) : a; };
+  auto chk = [&](const int32_t* ids, int cnt, int w) -> bool {
+    for (int i = 0; i < cnt * w; ++i)
+      if (ids[i] < 0 || ids[i] >= n) return false;
+    return true;
+  };
+  if (!chk(m->bond_ij, m->n_bonds, 2) || !chk(m->angle_ijk, m->n_angles, 3) ||
+      !chk(m->dih_ijkl, m->n_dihedrals, 4) || !chk(m->pairs14, m->n_pairs14, 2) ||
+      !chk(m->pairs, m->n_pairs, 2))
+    return fail(PXD_ERR_CONF, "receptor: term atom index out of range");
+
+  std::vector<float4> xyzq(n);
+  std::vector<float2> lj(n);
+  std::vector<int> fixed;
+  std::vector<float4> fxyzq;
+  std::vector<float2> flj;
+  for (int i = 0; i < n; ++i) {
+    xyzq[i] = make_float4((float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2],
+                          (float)m->charge[i]);
+    lj[i] = make_float2((float)m->sigma[i], (float)std::sqrt(m->epsilon[i]));
+    if (slot_of[i] < 0) {
+      fixed.push_back(i);
+      fxyzq.push_back(xyzq[i]);
+      flj.push_back(lj[i]);
+    }
+  }
+  // constant part: terms with no movable atom (FP64 on the host)
+  double e_const = 0.0;
+  std::vector<int2> bab;
+  std::vector<float2> bkr;
+  for (int b = 0; b < m->n_bonds; ++b) {
+    int i = m->bond_ij[2 * b], j = m->bond_ij[2 * b + 1];
+    if (slot_of[i] < 0 && slot_of[j] < 0) {
+      double r = h_dist(xyz + 3 * i, xyz + 3 * j) - m->bond_r0[b];
+      e_const += 0.5 * m->bond_k[b] * r * r;
+      continue;
+    }
+    bab.push_back(make_int2(ref(i), ref(j)));
+    bkr.push_back(make_float2((float)m->bond_k[b], (float)m->bond_r0[b]));
+  }
+  std::vector<int4> aabc;
+  std::vector<float2> akt;
+  for (int a = 0; a < m->n_angles; ++a) {
+    const int* id = m->angle_ijk + 3 * a;
+    if (slot_of[id[0]] < 0 && slot_of[id[1]] < 0 && slot_of[id[2]] < 0) {
+      const double *pi = xyz + 3 * id[0], *pj = xyz + 3 * id[1], *pk = xyz + 3 * id[2];
+      double v0[3], v1[3], c[3];
+      for (int d = 0; d < 3; ++d) {
+        v0[d] = pj[d] - pi[d];
+        v1[d] = pj[d] - pk[d];
+      }
+      c[0] = v0[1] * v1[2] - v0[2] * v1[1];
+      c[1] = v0[2] * v1[0] - v0[0] * v1[2];
+      c[2] = v0[0] * v1[1] - v0[1] * v1[0];
+      double th = std::atan2(std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]),
+                             v0[0] * v1[0] + v0[1] * v1[1] + v0[2] * v1[2] + 1e-20);
+      double dl = th - m->angle_t0[a];
+      e_const += 0.5 * m->angle_k[a] * dl * dl;
+      continue;
+    }
+    aabc.push_back(make_int4(ref(id[0]), ref(id[1]), ref(id[2]), 0));
+    akt.push_back(make_float2((float)m->angle_k[a], (float)m->angle_t0[a]));
+  }
+  std::vector<int4> dabcd;
+  std::vector<uint32_t> dper;
+  std::vector<float4> dkp;
+  int bad = 0;
+  for (int t = 0; t < m->n_dihedrals; ++t) {
+    const int* id = m->dih_ijkl + 4 * t;
+    const int32_t* per = m->dih_n + 6 * t;
+    if (slot_of[id[0]] < 0 && slot_of[id[1]] < 0 && slot_of[id[2]] < 0 && slot_of[id[3]] < 0) {
+      double th = h_dihedral(xyz + 3 * id[0], xyz + 3 * id[1], xyz + 3 * id[2], xyz + 3 * id[3]);
+      for (int k = 0; k < 6 && per[k] != 0; ++k)
+        e_const += m->dih_k[6 * t + k] * (1.0 + std::cos(th * per[k] - m->dih_phase[6 * t + k]));
+      continue;
+    }
+    dabcd.push_back(make_int4(ref(id[0]), ref(id[1]), ref(id[2]), ref(id[3])));
+    dper.push_back(pack_periods(per, &bad));
+    float4 kp[3];
+    pack_kp(m->dih_k + 6 * t, m->dih_phase + 6 * t, per, kp);
+    dkp.insert(dkp.end(), kp, kp + 3);
+  }
+  if (bad) return fail(PXD_ERR_CONF, "receptor: dihedral periodicity outside 0..15");
+  // non-bonded pairs
+  std::vector<std::vector<float4>> seg_pos(M);
+  std::vector<std::vector<float2>> seg_lj(M);
+  std::vector<int4> mm;
+  for (int pass = 0; pass < 2; ++pass) {
+    int cnt = pass == 0 ? m->n_pairs14 : m->n_pairs;
+    const int32_t* ij = pass == 0 ? m->pairs14 : m->pairs;
+    double cs = pass == 0 ? 0.833333333 : 1.0, vs = pass == 0 ? 0.5 : 1.0;
+    for (int p = 0; p < cnt; ++p) {
+      int i = ij[2 * p], j = ij[2 * p + 1];
+      int si = slot_of[i], sj = slot_of[j];
+      if (si < 0 && sj < 0) {
+        double sig = (m->sigma[i] + m->sigma[j]) * 0.5;
+        double e4 = std::sqrt(m->epsilon[i] * m->epsilon[j]) * vs * 4.0;
+        e_const += h_pair(m->charge[i] * m->charge[j] * cs, sig, e4, h_dist(xyz + 3 * i, xyz + 3 * j));
+      } else if (si >= 0 && sj >= 0) {
+        mm.push_back(make_int4(si, sj, pass == 0 ? 1 : 0, 0));
+      } else {
+        int mv = si >= 0 ? si : sj, fx = si >= 0 ? j : i;
+        seg_pos[mv].push_back(make_float4((float)xyz[3 * fx], (float)xyz[3 * fx + 1],
+                                          (float)xyz[3 * fx + 2], (float)(m->charge[fx] * cs)));
+        seg_lj[mv].push_back(make_float2((float)m->sigma[fx],
+                                         (float)(std::sqrt(m->epsilon[fx]) * 4.0 * vs)));
+      }
+    }
+  }
+  std::vector<int> rp_seg(M + 1, 0);
+  std::vector<float4> rp_pos;
+  std::vector<float2> rp_lj;
+  for (int s = 0; s < M; ++s) {
+    rp_pos.insert(rp_pos.end(), seg_pos[s].begin(), seg_pos[s].end());
+    rp_lj.insert(rp_lj.end(), seg_lj[s].begin(), seg_lj[s].end());
+    while (rp_pos.size() % kWave) {  // inert padding: no charge, no LJ well
+      rp_pos.push_back(make_float4(1e4f, 1e4f, 1e4f, 0.f));
+      rp_lj.push_back(make_float2(0.f, 0.f));
+    }
+    rp_seg[s + 1] = (int)rp_pos.size();
+  }
+  std::vector<float4> lorg(nl), lax(nl);
+  for (int k = 0; k < nl; ++k) {
+    lorg[k] = make_float4((float)lv->origin[3 * k], (float)lv->origin[3 * k + 1],
+                          (float)lv->origin[3 * k + 2], 0.f);
+    lax[k] = make_float4((float)lv->axis[3 * k], (float)lv->axis[3 * k + 1],
+                         (float)lv->axis[3 * k + 2], 0.f);
+  }
+  // pscore sets
+  auto ivec = [](const int32_t* p, int cnt) { return std::vector<int>(p, p + cnt); };
+  std::vector<int> hydro = ivec(m->hydrophobic, m->n_hydrophobic),
+                   cat = ivec(m->cation, m->n_cation), an = ivec(m->anion, m->n_anion),
+                   r5 = ivec(m->ring5, 5 * m->n_ring5), r6 = ivec(m->ring6, 6 * m->n_ring6),
+                   accc = ivec(m->acc_charged, m->n_acc_charged),
+                   accn = ivec(m->acc_neutral, m->n_acc_neutral);
+  std::vector<int2> donc(m->n_don_charged), donn(m->n_don_neutral);
+  for (int i = 0; i < m->n_don_charged; ++i)
+    donc[i] = make_int2(m->don_charged[2 * i], m->don_charged[2 * i + 1]);
+  for (int i = 0; i < m->n_don_neutral; ++i)
+    donn[i] = make_int2(m->don_neutral[2 * i], m->don_neutral[2 * i + 1]);
+  for (auto* v : {&hydro, &cat, &an, &r5, &r6, &accc, &accn})
+    for (int a : *v)
+      if (a < 0 || a >= n) return fail(PXD_ERR_CONF, "receptor: interaction-set atom out of range");
+
+  Packer P;
+  size_t o_xyzq = P.add(xyzq), o_lj = P.add(lj), o_fixed = P.add(fixed), o_fxyzq = P.add(fxyzq),
+         o_flj = P.add(flj), o_lorg = P.add(lorg), o_lax = P.add(lax),
+         o_movleaf = P.add(mov_leaf), o_movatom = P.add(mov_atom), o_bab = P.add(bab),
+         o_bkr = P.add(bkr), o_aabc = P.add(aabc), o_akt = P.add(akt), o_dabcd = P.add(dabcd),
+         o_dper = P.add(dper), o_dkp = P.add(dkp), o_rpseg = P.add(rp_seg),
+         o_rppos = P.add(rp_pos), o_rplj = P.add(rp_lj), o_mm = P.add(mm),
+         o_hydro = P.add(hydro), o_cat = P.add(cat), o_an = P.add(an), o_r5 = P.add(r5),
+         o_r6 = P.add(r6), o_donc = P.add(donc), o_donn = P.add(donn), o_accc = P.add(accc),
+         o_accn = P.add(accn), o_slot = P.add(slot_of);
+  pxd_receptor* r = new pxd_receptor();
+  hipError_t he = hipMalloc(&r->mem, P.host.size());
+  if (he != hipSuccess) {
+    delete r;
+    return fail(PXD_ERR_HIP, "receptor: hipMalloc: %s", hipGetErrorString(he));
+  }
+  he = hipMemcpy(r->mem, P.host.data(), P.host.size(), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    (void)hipFree(r->mem);
+    delete r;
+    return fail(PXD_ERR_HIP, "receptor: hipMemcpy: %s", hipGetErrorString(he));
+  }
+  r->mem_bytes = P.host.size();
+  DevReceptor& D = r->dev;
+  void* B = r->mem;
+  D.n_atoms = n;
+  D.xyzq = dptr<float4>(B, o_xyzq);
+  D.lj = dptr<float2>(B, o_lj);
+  D.n_fixed = (int)fixed.size();
+  D.fixed = dptr<int>(B, o_fixed);
+  D.n_leaves = nl;
+  D.n_mov = M;
+  D.leaf_origin = dptr<float4>(B, o_lorg);
+  D.leaf_axis = dptr<float4>(B, o_lax);
+  D.mov_leaf = dptr<int>(B, o_movleaf);
+  D.mov_atom = dptr<int>(B, o_movatom);
+  D.n_bonds = (int)bab.size();
+  D.bond_ab = dptr<int2>(B, o_bab);
+  D.bond_kr = dptr<float2>(B, o_bkr);
+  D.n_angles = (int)aabc.size();
+  D.angle_abc = dptr<int4>(B, o_aabc);
+  D.angle_kt = dptr<float2>(B, o_akt);
+  D.n_dih = (int)dabcd.size();
+  D.dih_abcd = dptr<int4>(B, o_dabcd);
+  D.dih_per = dptr<uint32_t>(B, o_dper);
+  D.dih_kp = dptr<float4>(B, o_dkp);
+  D.rp_seg = dptr<int>(B, o_rpseg);
+  D.rp_pos = dptr<float4>(B, o_rppos);
+  D.rp_lj = dptr<float2>(B, o_rplj);
+  D.n_mm = (int)mm.size();
+  D.mm = dptr<int4>(B, o_mm);
+  D.n_hydro = (int)hydro.size();
+  D.n_cation = (int)cat.size();
+  D.n_anion = (int)an.size();
+  D.n_ring5 = m->n_ring5;
+  D.n_ring6 = m->n_ring6;
+  D.n_don_c = m->n_don_charged;
+  D.n_don_n = m->n_don_neutral;
+  D.n_acc_c = m->n_acc_charged;
+  D.n_acc_n = m->n_acc_neutral;
+  D.hydro = dptr<int>(B, o_hydro);
+  D.cation = dptr<int>(B, o_cat);
+  D.anion = dptr<int>(B, o_an);
+  D.ring5 = dptr<int>(B, o_r5);
+  D.ring6 = dptr<int>(B, o_r6);
+  D.don_c = dptr<int2>(B, o_donc);
+  D.don_n = dptr<int2>(B, o_donn);
+  D.acc_c = dptr<int>(B, o_accc);
+  D.acc_n = dptr<int>(B, o_accn);
+  D.atom_mov_slot = dptr<int>(B, o_slot);
+  D.e_const = e_const;
+  D.fixed_xyzq = dptr<float4>(B, o_fxyzq);
+  D.fixed_lj = dptr<float2>(B, o_flj);
+  r->n_atoms = n;
+  r->n_leaves = nl;
+  r->n_mov = M;
+  r->h_xyz = nullptr;
+  *out = r;
+  return PXD_OK;
+}
+
+void pxd_receptor_destroy(pxd_receptor* r) {
+  if (!r) return;
+  (void)hipFree(r->mem);
+  delete r;
+}
+
+// ------------------------------------------------------------------ grid
+
+// the DevGrid record heads the grid allocation; maps start 256-byte aligned
+constexpr size_t kGridHeader = (sizeof(DevGrid) + 255) & ~size_t(255);
+
+static pxd_status grid_alloc(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                             int32_t n_types, double slope, pxd_grid** out) {
+  if (!r || !box || !out || (n_types > 0 && !types)) return fail(PXD_ERR_ARG, "grid: null argument");
+  if (n_types < 0 || n_types > PXD_NUM_VDW_TYPES) return fail(PXD_ERR_ARG, "grid: n_types=%d", n_types);
+  for (int d = 0; d < 3; ++d)
+    if (box->npoints[d] < 2) return fail(PXD_ERR_ARG, "grid: npoints[%d] < 2", d);
+  if (!(box->spacing > 0)) return fail(PXD_ERR_ARG, "grid: spacing must be > 0");
+  pxd_grid* g = new pxd_grid();
+  g->box = *box;
+  g->n_types = n_types;
+  g->receptor = r;
+  for (int t = 0; t < PXD_NUM_VDW_TYPES; ++t) g->dev.type_slot[t] = -1;
+  for (int t = 0; t < n_types; ++t) {
+    if (types[t] < 0 || types[t] >= PXD_NUM_VDW_TYPES) {
+      delete g;
+      return fail(PXD_ERR_ARG, "grid: type %d out of range", types[t]);
+    }
+    g->types[t] = types[t];
+    g->dev.type_slot[types[t]] = t + 1;
+  }
+  long long P = (long long)box->npoints[0] * box->npoints[1] * box->npoints[2];
+  size_t bytes = (size_t)(n_types + 1) * P * sizeof(float4);
+  hipError_t he = hipMalloc(&g->mem, bytes + kGridHeader);
+  if (he != hipSuccess) {
+    delete g;
+    return fail(PXD_ERR_HIP, "grid: hipMalloc(%zu): %s", bytes, hipGetErrorString(he));
+  }
+  DevGrid& D = g->dev;
+  for (int d = 0; d < 3; ++d) {
+    D.init[d] = (float)box->init_xyz[d];
+    D.np[d] = box->npoints[d];
+  }
+  D.spacing = (float)box->spacing;
+  D.inv_spacing = (float)(1.0 / box->spacing);
+  D.slope = (float)slope;
+  D.P = P;
+  D.maps = reinterpret_cast<float4*>(static_cast<char*>(g->mem) + kGridHeader);
+  *out = g;
+  return PXD_OK;
+}
+
+static DevGrid* grid_dev_ptr(const pxd_grid* g) { return static_cast<DevGrid*>(g->mem); }
+
+static pxd_status grid_sync_struct(pxd_grid* g) {
+  HIP_TRY(hipMemcpy(g->mem, &g->dev, sizeof(DevGrid), hipMemcpyHostToDevice));
+  return PXD_OK;
+}
+
+pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                          int32_t n_types, double slope, void* stream, pxd_grid** out) {
+  pxd_grid* g = nullptr;
+  pxd_status st = grid_alloc(r, box, types, n_types, slope, &g);
+  if (st != PXD_OK) return st;
+  std::vector<float2> tlj(n_types > 0 ? n_types : 1);
+  for (int t = 0; t < n_types; ++t)
+    tlj[t] = make_float2((float)kGaffSigma[types[t]], (float)(4.0 * std::sqrt(kGaffEpsilon[types[t]])));
+  float2* d_tlj = nullptr;
+  hipError_t he = hipMalloc(&d_tlj, tlj.size() * sizeof(float2));
+  if (he == hipSuccess) he = hipMemcpy(d_tlj, tlj.data(), tlj.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    pxd_grid_destroy(g);
+    return fail(PXD_ERR_HIP, "grid: %s", hipGetErrorString(he));
+  }
+  GridBuildArgs a;
+  a.atoms = r->dev.fixed_xyzq;
+  a.alj = r->dev.fixed_lj;
+  a.n_atoms = r->dev.n_fixed;
+  a.tlj = d_tlj;
+  a.n_types = n_types;
+  for (int d = 0; d < 3; ++d) {
+    a.init[d] = g->dev.init[d];
+    a.np[d] = g->dev.np[d];
+  }
+  a.spacing = g->dev.spacing;
+  a.P = g->dev.P;
+  a.maps = const_cast<float4*>(g->dev.maps);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  launch_grid_build(a, s);
+  he = hipGetLastError();
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  (void)hipFree(d_tlj);
+  if (he != hipSuccess) {
+    pxd_grid_destroy(g);
+    return fail(PXD_ERR_HIP, "grid build: %s", hipGetErrorString(he));
+  }
+  st = grid_sync_struct(g);
+  if (st != PXD_OK) {
+    pxd_grid_destroy(g);
+    return st;
+  }
+  *out = g;
+  return PXD_OK;
+}
+
+pxd_status pxd_grid_upload(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                           int32_t n_types, double slope, const float* maps, pxd_grid** out) {
+  if (!maps) return fail(PXD_ERR_ARG, "grid upload: null maps");
+  pxd_grid* g = nullptr;
+  pxd_status st = grid_alloc(r, box, types, n_types, slope, &g);
+  if (st != PXD_OK) return st;
+  size_t bytes = (size_t)(n_types + 1) * g->dev.P * sizeof(float4);
+  hipError_t he = hipMemcpy(const_cast<float4*>(g->dev.maps), maps, bytes, hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    pxd_grid_destroy(g);
+    return fail(PXD_ERR_HIP, "grid upload: %s", hipGetErrorString(he));
+  }
+  st = grid_sync_struct(g);
+  if (st != PXD_OK) {
+    pxd_grid_destroy(g);
+    return st;
+  }
+  *out = g;
+  return PXD_OK;
+}
+
+pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats) {
+  if (!g || !maps) return fail(PXD_ERR_ARG, "grid download: null argument");
+  size_t need = (size_t)(g->n_types + 1) * g->dev.P * 4;
+  if (n_floats < need) return fail(PXD_ERR_ARG, "grid download: buffer %zu < %zu floats", n_floats, need);
+  HIP_TRY(hipMemcpy(maps, g->dev.maps, need * sizeof(float), hipMemcpyDeviceToHost));
+  return PXD_OK;
+}
+
+pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types) {
+  if (!g) return fail(PXD_ERR_ARG, "grid info: null grid");
+  if (box) *box = g->box;
+  if (n_types) *n_types = g->n_types;
+  return PXD_OK;
+}
+
+pxd_status pxd_grid_set_slope(pxd_grid* g, double slope) {
+  if (!g) return fail(PXD_ERR_ARG, "grid slope: null grid");
+  g->dev.slope = (float)slope;
+  return grid_sync_struct(g);
+}
+
+void pxd_grid_destroy(pxd_grid* g) {
+  if (!g) return;
+  (void)hipFree(g->mem);
+  delete g;
+}
+
+// ------------------------------------------------------------------ ligands
+
+pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* trees, int32_t nlig,
+                                   pxd_ligand_batch** out) {
+  if (!ligs || nlig <= 0 || !out) return fail(PXD_ERR_ARG, "ligand batch: empty");
+  *out = nullptr;
+  std::vector<LigInfo> info(nlig);
+  std::vector<int> type;
+  std::vector<float> q;
+  std::vector<float2> lj;
+  std::vector<uint32_t> bij, aijk, dijkl, dper, pairs, tsijkl;
+  std::vector<float2> bkr, akt;
+  std::vector<float4> dkp, tscoef;
+  std::vector<int> frag, torsf, perm, sets, rings, tsmoff;
+  std::vector<int2> dons;
+  std::vector<int> natoms(nlig), ntors(nlig);
+  uint8_t used[PXD_NUM_VDW_TYPES] = {0};
+  for (int l = 0; l < nlig; ++l) {
+    const pxd_mol& m = ligs[l];
+    LigInfo& I = info[l];
+    memset(&I, 0, sizeof(I));
+    const int n = m.n_atoms;
+    if (n <= 0 || n > kMaxAtoms)
+      return fail(PXD_ERR_RANGE, "ligand %d: %d atoms (1..%d supported)", l, n, kMaxAtoms);
+    if (!m.vdw_type || !m.charge) return fail(PXD_ERR_CONF, "ligand %d: vdw_type/charge required", l);
+    auto ok = [&](const int32_t* ids, int cnt, int w) {
+      for (int i = 0; i < cnt * w; ++i)
+        if (ids[i] < 0 || ids[i] >= n) return false;
+      return true;
+    };
+    if (!ok(m.bond_ij, m.n_bonds, 2) || !ok(m.angle_ijk, m.n_angles, 3) ||
+        !ok(m.dih_ijkl, m.n_dihedrals, 4) || !ok(m.pairs14, m.n_pairs14, 2) ||
+        !ok(m.pairs, m.n_pairs, 2) || !ok(m.hydrophobic, m.n_hydrophobic, 1) ||
+        !ok(m.cation, m.n_cation, 1) || !ok(m.anion, m.n_anion, 1) ||
+        !ok(m.ring5, m.n_ring5, 5) || !ok(m.ring6, m.n_ring6, 6) ||
+        !ok(m.don_charged, m.n_don_charged, 2) || !ok(m.don_neutral, m.n_don_neutral, 2) ||
+        !ok(m.acc_charged, m.n_acc_charged, 1) || !ok(m.acc_neutral, m.n_acc_neutral, 1))
+      return fail(PXD_ERR_CONF, "ligand %d: term atom index out of range", l);
+    I.n_atoms = n;
+    I.atom_off = (int)type.size();
+    for (int i = 0; i < n; ++i) {
+      int t = m.vdw_type[i];
+      if (t < 0 || t >= PXD_NUM_VDW_TYPES)
+        return fail(PXD_ERR_CONF, "ligand %d: VDW type [%d] of atom#%d is illegal!", l, t, i);
+      used[t] = 1;
+      type.push_back(t);
+      q.push_back((float)m.charge[i]);
+      lj.push_back(make_float2((float)kGaffSigma[t], (float)std::sqrt(kGaffEpsilon[t])));
+    }
+    I.bond_off = (int)bij.size();
+    I.n_bonds = m.n_bonds;
+    for (int b = 0; b < m.n_bonds; ++b) {
+      bij.push_back(uint32_t(m.bond_ij[2 * b]) | uint32_t(m.bond_ij[2 * b + 1]) << 8);
+      bkr.push_back(make_float2((float)m.bond_k[b], (float)m.bond_r0[b]));
+    }
+    I.angle_off = (int)aijk.size();
+    I.n_angles = m.n_angles;
+    for (int a = 0; a < m.n_angles; ++a) {
+      const int* id = m.angle_ijk + 3 * a;
+      aijk.push_back(uint32_t(id[0]) | uint32_t(id[1]) << 8 | uint32_t(id[2]) << 16);
+      akt.push_back(make_float2((float)m.angle_k[a], (float)m.angle_t0[a]));
+    }
+    I.dih_off = (int)dijkl.size();
+    I.n_dih = m.n_dihedrals;
+    int bad = 0;
+    for (int t = 0; t < m.n_dihedrals; ++t) {
+      const int* id = m.dih_ijkl + 4 * t;
+      dijkl.push_back(uint32_t(id[0]) | uint32_t(id[1]) << 8 | uint32_t(id[2]) << 16 |
+                      uint32_t(id[3]) << 24);
+      dper.push_back(pack_periods(m.dih_n + 6 * t, &bad));
+      float4 kp[3];
+      pack_kp(m.dih_k + 6 * t, m.dih_phase + 6 * t, m.dih_n + 6 * t, kp);
+      dkp.insert(dkp.end(), kp, kp + 3);
+    }
+    if (bad) return fail(PXD_ERR_CONF, "ligand %d: dihedral periodicity outside 0..15", l);
+    I.pair_off = (int)pairs.size();
+    I.n_pairs = m.n_pairs14 + m.n_pairs;
+    for (int p = 0; p < m.n_pairs14; ++p)
+      pairs.push_back(uint32_t(m.pairs14[2 * p]) | uint32_t(m.pairs14[2 * p + 1]) << 8 | 1u << 16);
+    for (int p = 0; p < m.n_pairs; ++p)
+      pairs.push_back(uint32_t(m.pairs[2 * p]) | uint32_t(m.pairs[2 * p + 1]) << 8);
+    // fragment tree
+    I.frag_off = (int)torsf.size();
+    if (trees) {
+      const pxd_frag_tree& T = trees[l];
+      if (T.n_frags < 1 || T.n_frags > kMaxFrags)
+        return fail(PXD_ERR_RANGE, "ligand %d: %d fragments (1..%d)", l, T.n_frags, kMaxFrags);
+      I.n_frags = T.n_frags;
+      I.n_tors = T.n_torsions;
+      for (int f = 0; f < T.n_frags; ++f) {
+        int s = T.frag_start[f], e = T.frag_end[f], par = T.frag_parent[f];
+        if (s < 0 || e > n || s >= e || par >= T.n_frags)
+          return fail(PXD_ERR_CONF, "ligand %d: bad fragment %d", l, f);
+        frag.insert(frag.end(), {s, e, par, T.frag_bond[2 * f], T.frag_bond[2 * f + 1], 0});
+      }
+      for (int k = 0; k < T.n_torsions; ++k) torsf.push_back(T.tors_frag[k]);
+      torsf.resize(I.frag_off + T.n_frags, 0);
+      for (int i = 0; i < n; ++i) perm.push_back(T.perm[i]);
+    } else {
+      frag.insert(frag.end(), {0, n, -1, 0, 0, 0});
+      torsf.push_back(0);
+      I.n_frags = 1;
+      for (int i = 0; i < n; ++i) perm.push_back(i);
+    }
+    // pscore sets
+    auto put = [&](const int32_t* p, int cnt, int* off, int* num) {
+      *off = (int)sets.size();
+      *num = cnt;
+      sets.insert(sets.end(), p, p + cnt);
+    };
+    put(m.hydrophobic, m.n_hydrophobic, &I.hydro_off, &I.n_hydro);
+    put(m.cation, m.n_cation, &I.cation_off, &I.n_cation);
+    put(m.anion, m.n_anion, &I.anion_off, &I.n_anion);
+    put(m.acc_charged, m.n_acc_charged, &I.accc_off, &I.n_accc);
+    put(m.acc_neutral, m.n_acc_neutral, &I.accn_off, &I.n_accn);
+    if (m.n_ring5 + m.n_ring6 > 16)
+      return fail(PXD_ERR_RANGE, "ligand %d: more than 16 pi rings", l);
+    I.ring5_off = (int)rings.size();
+    I.n_ring5 = m.n_ring5;
+    rings.insert(rings.end(), m.ring5, m.ring5 + 5 * m.n_ring5);
+    I.ring6_off = (int)rings.size();
+    I.n_ring6 = m.n_ring6;
+    rings.insert(rings.end(), m.ring6, m.ring6 + 6 * m.n_ring6);
+    I.donc_off = (int)dons.size();
+    I.n_donc = m.n_don_charged;
+    for (int i = 0; i < m.n_don_charged; ++i)
+      dons.push_back(make_int2(m.don_charged[2 * i], m.don_charged[2 * i + 1]));
+    I.donn_off = (int)dons.size();
+    I.n_donn = m.n_don_neutral;
+    for (int i = 0; i < m.n_don_neutral; ++i)
+      dons.push_back(make_int2(m.don_neutral[2 * i], m.don_neutral[2 * i + 1]));
+    I.ts_off = (int)tscoef.size();
+    I.n_ts = m.n_tstrain;
+    I.tsm_off = (int)tsmoff.size();
+    for (int p = 0; p < m.n_tstrain; ++p) {
+      const double* c = m.tstrain_coef + 4 * p;
+      tscoef.push_back(make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[3]));
+      tsmoff.push_back((int)tsijkl.size());
+      for (int k = m.tstrain_match_off[p]; k < m.tstrain_match_off[p + 1]; ++k) {
+        const int* a = m.tstrain_ijkl + 4 * k;
+        for (int d = 0; d < 4; ++d)
+          if (a[d] < 0 || a[d] >= n) return fail(PXD_ERR_CONF, "ligand %d: torsion-strain atom out of range", l);
+        tsijkl.push_back(uint32_t(a[0]) | uint32_t(a[1]) << 8 | uint32_t(a[2]) << 16 | uint32_t(a[3]) << 24);
+      }
+    }
+    tsmoff.push_back((int)tsijkl.size());
+    I.n_rot = m.n_rotatable;
+    natoms[l] = n;
+    ntors[l] = I.n_tors;
+  }
+  Packer P;
+  size_t o_info = P.add(info), o_type = P.add(type), o_q = P.add(q), o_lj = P.add(lj),
+         o_bij = P.add(bij), o_bkr = P.add(bkr), o_aijk = P.add(aijk), o_akt = P.add(akt),
+         o_dijkl = P.add(dijkl), o_dper = P.add(dper), o_dkp = P.add(dkp), o_pairs = P.add(pairs),
+         o_frag = P.add(frag), o_torsf = P.add(torsf), o_perm = P.add(perm), o_sets = P.add(sets),
+         o_rings = P.add(rings), o_dons = P.add(dons), o_tsc = P.add(tscoef),
+         o_tsm = P.add(tsmoff), o_tsi = P.add(tsijkl);
+  pxd_ligand_batch* b = new pxd_ligand_batch();
+  hipError_t he = hipMalloc(&b->mem, P.host.size());
+  if (he == hipSuccess) he = hipMemcpy(b->mem, P.host.data(), P.host.size(), hipMemcpyHostToDevice);
+  if (he != hipSuccess) {
+    if (b->mem) (void)hipFree(b->mem);
+    delete b;
+    return fail(PXD_ERR_HIP, "ligand batch: %s", hipGetErrorString(he));
+  }
+  void* B = b->mem;
+  DevLigands& D = b->dev;
+  D.n = nlig;
+  D.info = dptr<LigInfo>(B, o_info);
+  D.type = dptr<int>(B, o_type);
+  D.q = dptr<float>(B, o_q);
+  D.lj = dptr<float2>(B, o_lj);
+  D.bond_ij = dptr<uint32_t>(B, o_bij);
+  D.bond_kr = dptr<float2>(B, o_bkr);
+  D.angle_ijk = dptr<uint32_t>(B, o_aijk);
+  D.angle_kt = dptr<float2>(B, o_akt);
+  D.dih_ijkl = dptr<uint32_t>(B, o_dijkl);
+  D.dih_per = dptr<uint32_t>(B, o_dper);
+  D.dih_kp = dptr<float4>(B, o_dkp);
+  D.pairs = dptr<uint32_t>(B, o_pairs);
+  D.frag = dptr<int>(B, o_frag);
+  D.tors_frag = dptr<int>(B, o_torsf);
+  D.perm = dptr<int>(B, o_perm);
+  D.sets = dptr<int>(B, o_sets);
+  D.rings = dptr<int>(B, o_rings);
+  D.dons = dptr<int2>(B, o_dons);
+  D.ts_coef = dptr<float4>(B, o_tsc);
+  D.ts_moff = dptr<int>(B, o_tsm);
+  D.ts_ijkl = dptr<uint32_t>(B, o_tsi);
+  b->mem_bytes = P.host.size();
+  b->n = nlig;
+  b->h_natoms = new int[nlig];
+  b->h_ntors = new int[nlig];
+  memcpy(b->h_natoms, natoms.data(), sizeof(int) * nlig);
+  memcpy(b->h_ntors, ntors.data(), sizeof(int) * nlig);
+  memcpy(b->type_used, used, sizeof(used));
+  *out = b;
+  return PXD_OK;
+}
+
+void pxd_ligand_batch_destroy(pxd_ligand_batch* b) {
+  if (!b) return;
+  (void)hipFree(b->mem);
+  delete[] b->h_natoms;
+  delete[] b->h_ntors;
+  delete b;
+}
+
+pxd_status pxd_ligand_batch_natoms(const pxd_ligand_batch* b, int32_t i, int32_t* out) {
+  if (!b || i < 0 || i >= b->n || !out) return fail(PXD_ERR_ARG, "natoms: bad index");
+  *out = b->h_natoms[i];
+  return PXD_OK;
+}
+
+// ------------------------------------------------------------------ scoring
+
+static pxd_status check_grid_types(const pxd_grid* g, const pxd_ligand_batch* b) {
+  if (!g) return PXD_OK;
+  for (int t = 0; t < PXD_NUM_VDW_TYPES; ++t)
+    if (b->type_used[t] && g->dev.type_slot[t] < 0)
+      return fail(PXD_ERR_STATE, "ligand type %d has no map in the grid", t);
+  return PXD_OK;
+}
+
+pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* g,
+                                const pxd_ligand_batch* b, int32_t n_poses,
+                                const int32_t* pose_lig, const int32_t* pose_atom_off,
+                                const float* xyz, const float* tors, double* energy,
+                                float* lig_grad, float* tors_grad, void* stream) {
+  if (!r || !b) return fail(PXD_ERR_ARG, "score: null receptor/batch");
+  if (n_poses < 0) return fail(PXD_ERR_ARG, "score: n_poses < 0");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !energy || !lig_grad)
+    return fail(PXD_ERR_ARG, "score: null pose array");
+  if (r->n_leaves > 0 && tors_grad == nullptr)
+    return fail(PXD_ERR_ARG, "score: tors_grad required when the receptor has leaves");
+  pxd_status st = check_grid_types(g, b);
+  if (st != PXD_OK) return st;
+  ScoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, energy, lig_grad, tors_grad};
+  launch_score(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return PXD_OK;
+}
+
+pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                           const pxd_lbfgs_params* prm, int32_t n_poses, const int32_t* pose_lig,
+                           const int32_t* pose_atom_off, float* xyz, float* tors,
+                           double* energy, int32_t* nevals, int32_t* converged, void* stream) {
+  if (!r || !b || !prm) return fail(PXD_ERR_ARG, "lbfgs: null argument");
+  if (n_poses < 0) return fail(PXD_ERR_ARG, "lbfgs: n_poses < 0");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !energy || !nevals || !converged)
+    return fail(PXD_ERR_ARG, "lbfgs: null pose array");
+  if (r->n_leaves > 0 && !tors) return fail(PXD_ERR_ARG, "lbfgs: tors required (receptor has leaves)");
+  if (prm->history < 1 || prm->history > kMaxHistory)
+    return fail(PXD_ERR_RANGE, "lbfgs: history %d outside 1..%d", prm->history, kMaxHistory);
+  if (prm->max_iter < 0 || prm->max_eval < 0 || prm->max_ls < 0)
+    return fail(PXD_ERR_ARG, "lbfgs: negative iteration limits");
+  pxd_status st = check_grid_types(g, b);
+  if (st != PXD_OK) return st;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int dev = 0, ncu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int n_slots = std::min(n_poses, std::max(1, ncu) * 8);
+  size_t per_slot = (size_t)2 * prm->history * kWave * kLbfgsVpl;
+  std::lock_guard<std::mutex> lock(g_ws.mu);
+  if (g_ws.hist_floats < per_slot * n_slots) {
+    if (g_ws.hist) HIP_TRY(hipFree(g_ws.hist));
+    g_ws.hist = nullptr;
+    g_ws.hist_floats = 0;
+    HIP_TRY(hipMalloc(&g_ws.hist, per_slot * n_slots * sizeof(float)));
+    g_ws.hist_floats = per_slot * n_slots;
+  }
+  if (!g_ws.counter) HIP_TRY(hipMalloc(&g_ws.counter, sizeof(int) * 64));
+  HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int), s));
+  LbfgsArgs a;
+  a.prm = {prm->history, prm->max_iter, prm->max_eval, prm->max_ls, prm->lr, prm->tol_grad,
+           prm->tol_change, prm->c1, prm->c2, prm->ls_tol_change, prm->gtd_threshold};
+  a.n_poses = n_poses;
+  a.pose_lig = pose_lig;
+  a.pose_off = pose_atom_off;
+  a.xyz = xyz;
+  a.tors = tors;
+  a.energy = energy;
+  a.nevals = nevals;
+  a.converged = converged;
+  a.counter = g_ws.counter;
+  a.hist = g_ws.hist;
+  launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
+  HIP_TRY(hipGetLastError());
+  // the shared workspace may be reused by the next call: finish this one
+  HIP_TRY(hipStreamSynchronize(s));
+  return PXD_OK;
+}
+
+pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
+                            const pxd_score_params* sp, int32_t n_poses, const int32_t* pose_lig,
+                            const int32_t* pose_atom_off, const float* xyz, const float* tors,
+                            double* score, double* terms, void* stream) {
+  if (!r || !b || !sp) return fail(PXD_ERR_ARG, "pscore: null argument");
+  if (n_poses < 0) return fail(PXD_ERR_ARG, "pscore: n_poses < 0");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !score) return fail(PXD_ERR_ARG, "pscore: null pose array");
+  if (r->dev.n_ring5 + r->dev.n_ring6 > 256)
+    return fail(PXD_ERR_RANGE, "pscore: receptor has more than 256 pi rings");
+  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, terms};
+  launch_pscore(r->dev, b->dev, *sp, a, static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return PXD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// pxd_device.hpp — device functions of the docking hot path (gfx950, wave64).
+//
+// One wavefront evaluates one pose: lanes run in parallel over atoms and
+// over the term lists of the pose's ligand; per-pose coordinates, gradients
+// and parameters sit in LDS; energies accumulate per lane in FP64 and are
+// reduced across the wave.  Every function names the reference code it
+// restates (paths relative to the Protenix-Dock tree).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pxd_internal.hpp"
+
+namespace pxd {
+
+// ---------------------------------------------------------------- reductions
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+__device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
+
+// ---------------------------------------------------------------- geometry
+
+// math.h:176-178
+__device__ inline float safe_div(float a, float b) { return a / (b + kEps); }
+
+// math.cpp:62-94: Rodrigues matrix and d/dtheta
+__device__ inline void rot_axis_grad(float kx, float ky, float kz, float t, float* m,
+                                     float* j) {
+  float s, c;
+  sincosf(t, &s, &c);
+  float o = 1.f - c;
+  m[0] = c + kx * kx * o;
+  m[1] = kx * ky * o - kz * s;
+  m[2] = ky * s + kx * kz * o;
+  m[3] = kz * s + kx * ky * o;
+  m[4] = c + ky * ky * o;
+  m[5] = ky * kz * o - kx * s;
+  m[6] = kx * kz * o - ky * s;
+  m[7] = kx * s + ky * kz * o;
+  m[8] = c + kz * kz * o;
+  j[0] = -s + kx * kx * s;
+  j[1] = -kz * c + kx * ky * s;
+  j[2] = ky * c + kx * kz * s;
+  j[3] = kz * c + kx * ky * s;
+  j[4] = -s + ky * ky * s;
+  j[5] = -kx * c + ky * kz * s;
+  j[6] = -ky * c + kx * kz * s;
+  j[7] = kx * c + ky * kz * s;
+  j[8] = -s + kz * kz * s;
+}
+
+__device__ inline void rot_axis(float kx, float ky, float kz, float t, float* m) {
+  float s, c;
+  sincosf(t, &s, &c);
+  float o = 1.f - c;
+  m[0] = c + kx * kx * o;
+  m[1] = kx * ky * o - kz * s;
+  m[2] = ky * s + kx * kz * o;
+  m[3] = kz * s + kx * ky * o;
+  m[4] = c + ky * ky * o;
+  m[5] = ky * kz * o - kx * s;
+  m[6] = kx * kz * o - ky * s;
+  m[7] = kx * s + ky * kz * o;
+  m[8] = c + kz * kz * o;
+}
+
+__device__ inline void mv3(const float* m, const float* v, float* o) {
+  o[0] = m[0] * v[0] + m[1] * v[1] + m[2] * v[2];
+  o[1] = m[3] * v[0] + m[4] * v[1] + m[5] * v[2];
+  o[2] = m[6] * v[0] + m[7] * v[1] + m[8] * v[2];
+}
+
+__device__ inline void cross3(const float* a, const float* b, float* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ inline float dot3(const float* a, const float* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// math.cpp:96-110
+__device__ inline float angle_grad(const float* pi, const float* pj, const float* pk,
+                                   float* gi, float* gk) {
+  float v0[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+  float v1[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
+  float cr[3];
+  cross3(v0, v1, cr);
+  float rp = sqrtf(dot3(cr, cr));
+  cross3(v0, cr, gi);
+  cross3(cr, v1, gk);
+  float s0 = -safe_div(1.f, dot3(v0, v0) * rp);
+  float s1 = -safe_div(1.f, dot3(v1, v1) * rp);
+  for (int d = 0; d < 3; ++d) {
+    gi[d] *= s0;
+    gk[d] *= s1;
+  }
+  return atan2f(rp, dot3(v0, v1) + kEps);
+}
+
+// math.cpp:130-167
+__device__ inline float dihedral_grad(const float* pi, const float* pj, const float* pk,
+                                      const float* pl, float* gi, float* gj, float* gk,
+                                      float* gl) {
+  float rij[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+  float rkj[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
+  float rkl[3] = {pl[0] - pk[0], pl[1] - pk[1], pl[2] - pk[2]};
+  float m[3], n[3], w[3];
+  cross3(rij, rkj, m);
+  cross3(rkj, rkl, n);
+  cross3(m, n, w);
+  float wlen = sqrtf(dot3(w, w));
+  float phi = atan2f(wlen, dot3(m, n) + kEps);
+  float theta = dot3(rij, n) < 0.f ? phi : -phi;
+  float nrkj2 = dot3(rkj, rkj);
+  float nrkj_1 = safe_div(1.f, sqrtf(nrkj2));
+  float nrkj_2 = nrkj_1 * nrkj_1;
+  float nrkj = nrkj2 * nrkj_1;
+  float a = -nrkj / dot3(m, m);
+  float b = nrkj / dot3(n, n);
+  float p = dot3(rij, rkj) * nrkj_2;
+  float q = dot3(rkj, rkl) * nrkj_2;
+  for (int d = 0; d < 3; ++d) {
+    gi[d] = -a * m[d];
+    gl[d] = -b * n[d];
+  }
+  for (int d = 0; d < 3; ++d) {
+    float sv = p * gi[d] - q * gl[d];
+    gj[d] = -(gi[d] - sv);
+    gk[d] = -(gl[d] + sv);
+  }
+  return theta;
+}
+
+// math.cpp:112-128 (angle only)
+__device__ inline float dihedral(const float* pi, const float* pj, const float* pk,
+                                 const float* pl) {
+  float rij[3] = {pj[0] - pi[0], pj[1] - pi[1], pj[2] - pi[2]};
+  float rkj[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
+  float rkl[3] = {pl[0] - pk[0], pl[1] - pk[1], pl[2] - pk[2]};
+  float m[3], n[3], w[3];
+  cross3(rij, rkj, m);
+  cross3(rkj, rkl, n);
+  cross3(m, n, w);
+  float phi = atan2f(sqrtf(dot3(w, w)), dot3(m, n) + kEps);
+  return dot3(rij, n) < 0.f ? phi : -phi;
+}
+
+// Coulomb + LJ of one pair (pair.h:25-60) with r = x_j - x_i.  qq carries
+// q_i q_j scale, e4 carries sqrt(eps_i eps_j) * 4 * scale.  Returns the energy
+// and writes de_dr_r (the gradient on i is -r * de_dr_r, on j +r * de_dr_r).
+__device__ inline float pair_energy(float rx, float ry, float rz, float qq, float sig,
+                                    float e4, float* de_dr_r) {
+  float d2 = rx * rx + ry * ry + rz * rz;
+  float d = sqrtf(d2);
+  float ec = safe_div(qq, d) * kCoulomb;
+  float dc = -safe_div(qq, d2 * d) * kCoulomb;
+  float u = safe_div(sig, d);
+  float u2 = u * u;
+  float u6 = u2 * u2 * u2;
+  float u12 = u6 * u6;
+  float irsqr = safe_div(1.f, d2);
+  float dv = (-12.f * u12 + 6.f * u6) * irsqr * e4;
+  *de_dr_r = dc + dv;
+  return ec + (u12 - u6) * e4;
+}
+
+// ---------------------------------------------------------------- per-pose LDS
+
+struct PoseLds {
+  float* xyz;    // [kMaxAtoms*3]
+  float* grad;   // [kMaxAtoms*3]
+  float* q;      // [kMaxAtoms]
+  float2* lj;    // [kMaxAtoms] sigma, sqrt(eps)
+  int* slot;     // [kMaxAtoms] grid map index
+  float* mov;    // [kMaxMovable*3]
+  float* movg;   // [kMaxMovable*3]
+  float* jac;    // [kMaxMovable*3]
+  float* tors;   // [kMaxLeaves]
+  float* torsg;  // [kMaxLeaves]
+};
+
+#define PXD_POSE_LDS_FLOATS \
+  (pxd::kMaxAtoms * 3 * 2 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 9 + pxd::kMaxLeaves * 2)
+
+__device__ inline PoseLds carve_pose_lds(float* base) {
+  PoseLds s;
+  s.xyz = base;
+  s.grad = s.xyz + kMaxAtoms * 3;
+  s.q = s.grad + kMaxAtoms * 3;
+  s.lj = reinterpret_cast<float2*>(s.q + kMaxAtoms);
+  s.slot = reinterpret_cast<int*>(s.q + kMaxAtoms * 3);
+  s.mov = s.q + kMaxAtoms * 4;
+  s.movg = s.mov + kMaxMovable * 3;
+  s.jac = s.movg + kMaxMovable * 3;
+  s.tors = s.jac + kMaxMovable * 3;
+  s.torsg = s.tors + kMaxLeaves;
+  return s;
+}
+
+// Per-atom parameters of a ligand into LDS (pose independent).
+__device__ inline void load_ligand_params(const DevLigands& L, const LigInfo& li,
+                                          const DevGrid* G, PoseLds& s) {
+  for (int a = lane_id(); a < li.n_atoms; a += kWave) {
+    int g = li.atom_off + a;
+    s.q[a] = L.q[g];
+    s.lj[a] = L.lj[g];
+    s.slot[a] = G ? G->type_slot[L.type[g]] : 0;
+  }
+}
+
+// Receptor atom position for a term reference (fixed atom or movable slot).
+__device__ inline void rec_pos(const DevReceptor& R, const PoseLds& s, int ref, float* p) {
+  if (ref >= 0) {
+    float4 v = R.xyzq[ref];
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+  } else {
+    int m = -ref - 1;
+    p[0] = s.mov[3 * m];
+    p[1] = s.mov[3 * m + 1];
+    p[2] = s.mov[3 * m + 2];
+  }
+}
+
+__device__ inline void rec_grad_add(PoseLds& s, int ref, const float* g, float f) {
+  if (ref < 0) {
+    int m = -ref - 1;
+    atomicAdd(&s.movg[3 * m], g[0] * f);
+    atomicAdd(&s.movg[3 * m + 1], g[1] * f);
+    atomicAdd(&s.movg[3 * m + 2], g[2] * f);
+  }
+}
+
+__device__ inline void lig_grad_add(PoseLds& s, int a, const float* g, float f) {
+  atomicAdd(&s.grad[3 * a], g[0] * f);
+  atomicAdd(&s.grad[3 * a + 1], g[1] * f);
+  atomicAdd(&s.grad[3 * a + 2], g[2] * f);
+}
+
+// Fourier series of one dihedral (interaction.cpp:76-102): energy and dE/dtheta.
+__device__ inline float fourier(uint32_t per, const float4* kp, float theta, float* dedt) {
+  float4 a = kp[0], b = kp[1], c = kp[2];
+  float k[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+  float ph[6] = {b.z, b.w, c.x, c.y, c.z, c.w};
+  float e = 0.f, de = 0.f;
+  for (int n = 0; n < PXD_DIH_TERMS; ++n) {
+    int pn = (per >> (4 * n)) & 15;
+    if (pn == 0) break;
+    float dl = theta * pn - ph[n];
+    float sn, cs;
+    sincosf(dl, &sn, &cs);
+    e += k[n] * (1.f + cs);
+    de += k[n] * pn * sn;
+  }
+  *dedt = de;
+  return e;
+}
+
+// ---------------------------------------------------------------- energy terms
+
+// receptor_torsion2xyz (interaction.cpp:316-336): movable positions and
+// Jacobians from the pose's receptor torsions (s.tors).
+__device__ inline void receptor_torsion2xyz(const DevReceptor& R, PoseLds& s) {
+  for (int m = lane_id(); m < R.n_mov; m += kWave) {
+    int leaf = R.mov_leaf[m];
+    float4 o = R.leaf_origin[leaf], ax = R.leaf_axis[leaf];
+    float4 p0 = R.xyzq[R.mov_atom[m]];
+    float rm[9], jm[9];
+    rot_axis_grad(ax.x, ax.y, ax.z, s.tors[leaf], rm, jm);
+    float t[3] = {p0.x - o.x, p0.y - o.y, p0.z - o.z}, r[3], j[3];
+    mv3(rm, t, r);
+    mv3(jm, t, j);
+    s.mov[3 * m] = r[0] + o.x;
+    s.mov[3 * m + 1] = r[1] + o.y;
+    s.mov[3 * m + 2] = r[2] + o.z;
+    s.jac[3 * m] = j[0];
+    s.jac[3 * m + 1] = j[1];
+    s.jac[3 * m + 2] = j[2];
+    s.movg[3 * m] = s.movg[3 * m + 1] = s.movg[3 * m + 2] = 0.f;
+  }
+}
+
+// Receptor intra terms that touch movable atoms (interaction.cpp:27-168 on
+// the pruned receptor lists).  Gradients on fixed atoms are dropped: only the
+// movable atoms feed the torsion chain rule (interaction.cpp:338-350).
+__device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
+  double e = 0.0;
+  const int lane = lane_id();
+  for (int b = lane; b < R.n_bonds; b += kWave) {
+    int2 ab = R.bond_ab[b];
+    float2 kr = R.bond_kr[b];
+    float pa[3], pb[3];
+    rec_pos(R, s, ab.x, pa);
+    rec_pos(R, s, ab.y, pb);
+    float d[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+    float r = sqrtf(dot3(d, d));
+    float dl = r - kr.y;
+    e += 0.5f * kr.x * dl * dl;
+    float de = kr.x * (1.f - safe_div(kr.y, r));
+    rec_grad_add(s, ab.x, d, -de);
+    rec_grad_add(s, ab.y, d, de);
+  }
+  for (int a = lane; a < R.n_angles; a += kWave) {
+    int4 id = R.angle_abc[a];
+    float2 kt = R.angle_kt[a];
+    float pi[3], pj[3], pk[3], fi[3], fk[3];
+    rec_pos(R, s, id.x, pi);
+    rec_pos(R, s, id.y, pj);
+    rec_pos(R, s, id.z, pk);
+    float th = angle_grad(pi, pj, pk, fi, fk);
+    float dl = th - kt.y;
+    e += 0.5f * kt.x * dl * dl;
+    float de = kt.x * dl;
+    float fj[3] = {fi[0] + fk[0], fi[1] + fk[1], fi[2] + fk[2]};
+    rec_grad_add(s, id.x, fi, de);
+    rec_grad_add(s, id.z, fk, de);
+    rec_grad_add(s, id.y, fj, -de);
+  }
+  for (int t = lane; t < R.n_dih; t += kWave) {
+    int4 id = R.dih_abcd[t];
+    float p[4][3], g[4][3];
+    rec_pos(R, s, id.x, p[0]);
+    rec_pos(R, s, id.y, p[1]);
+    rec_pos(R, s, id.z, p[2]);
+    rec_pos(R, s, id.w, p[3]);
+    float th = dihedral_grad(p[0], p[1], p[2], p[3], g[0], g[1], g[2], g[3]);
+    float dedt;
+    e += fourier(R.dih_per[t], R.dih_kp + 3 * t, th, &dedt);
+    rec_grad_add(s, id.x, g[0], -dedt);
+    rec_grad_add(s, id.y, g[1], -dedt);
+    rec_grad_add(s, id.z, g[2], -dedt);
+    rec_grad_add(s, id.w, g[3], -dedt);
+  }
+  // movable-vs-movable pairs
+  for (int k = lane; k < R.n_mm; k += kWave) {
+    int4 mm = R.mm[k];
+    float cs = mm.z ? kScale14Coul : 1.f, vs = mm.z ? kScale14Vdw : 1.f;
+    float4 qa = R.xyzq[R.mov_atom[mm.x]], qb = R.xyzq[R.mov_atom[mm.y]];
+    float2 la = R.lj[R.mov_atom[mm.x]], lb = R.lj[R.mov_atom[mm.y]];
+    float r[3] = {s.mov[3 * mm.y] - s.mov[3 * mm.x], s.mov[3 * mm.y + 1] - s.mov[3 * mm.x + 1],
+                  s.mov[3 * mm.y + 2] - s.mov[3 * mm.x + 2]};
+    float de;
+    e += pair_energy(r[0], r[1], r[2], qa.w * qb.w * cs, (la.x + lb.x) * 0.5f,
+                     la.y * lb.y * 4.f * vs, &de);
+    rec_grad_add(s, mov_ref(mm.x), r, -de);
+    rec_grad_add(s, mov_ref(mm.y), r, de);
+  }
+  __syncthreads();
+  // movable-vs-fixed pairs: per-slot segments padded to the wave width, the
+  // slot's gradient stays in registers and is reduced once per segment
+  for (int m = 0; m < R.n_mov; ++m) {
+    float px = s.mov[3 * m], py = s.mov[3 * m + 1], pz = s.mov[3 * m + 2];
+    int atom = R.mov_atom[m];
+    float qm = R.xyzq[atom].w;
+    float2 lm = R.lj[atom];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    double em = 0.0;
+    for (int p = R.rp_seg[m] + lane; p < R.rp_seg[m + 1]; p += kWave) {
+      float4 pj = R.rp_pos[p];
+      float2 lj = R.rp_lj[p];
+      float rx = pj.x - px, ry = pj.y - py, rz = pj.z - pz, de;
+      em += pair_energy(rx, ry, rz, qm * pj.w, (lm.x + lj.x) * 0.5f, lm.y * lj.y, &de);
+      gx -= rx * de;
+      gy -= ry * de;
+      gz -= rz * de;
+    }
+    e += em;
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    if (lane == 0) {
+      s.movg[3 * m] += gx;
+      s.movg[3 * m + 1] += gy;
+      s.movg[3 * m + 2] += gz;
+    }
+  }
+  return e;
+}
+
+// intra_molecular_interactions::put_gradients on the ligand (interaction.cpp:170-196)
+__device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, PoseLds& s) {
+  double e = 0.0;
+  const int lane = lane_id();
+  for (int b = lane; b < li.n_bonds; b += kWave) {
+    uint32_t ij = L.bond_ij[li.bond_off + b];
+    float2 kr = L.bond_kr[li.bond_off + b];
+    int i = ij & 255, j = (ij >> 8) & 255;
+    float d[3] = {s.xyz[3 * j] - s.xyz[3 * i], s.xyz[3 * j + 1] - s.xyz[3 * i + 1],
+                  s.xyz[3 * j + 2] - s.xyz[3 * i + 2]};
+    float r = sqrtf(dot3(d, d));
+    float dl = r - kr.y;
+    e += 0.5f * kr.x * dl * dl;
+    float de = kr.x * (1.f - safe_div(kr.y, r));
+    lig_grad_add(s, i, d, -de);
+    lig_grad_add(s, j, d, de);
+  }
+  for (int a = lane; a < li.n_angles; a += kWave) {
+    uint32_t ijk = L.angle_ijk[li.angle_off + a];
+    float2 kt = L.angle_kt[li.angle_off + a];
+    int i = ijk & 255, j = (ijk >> 8) & 255, k = (ijk >> 16) & 255;
+    float fi[3], fk[3];
+    float th = angle_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, fi, fk);
+    float dl = th - kt.y;
+    e += 0.5f * kt.x * dl * dl;
+    float de = kt.x * dl;
+    float fj[3] = {fi[0] + fk[0], fi[1] + fk[1], fi[2] + fk[2]};
+    lig_grad_add(s, i, fi, de);
+    lig_grad_add(s, k, fk, de);
+    lig_grad_add(s, j, fj, -de);
+  }
+  for (int t = lane; t < li.n_dih; t += kWave) {
+    uint32_t q = L.dih_ijkl[li.dih_off + t];
+    int i = q & 255, j = (q >> 8) & 255, k = (q >> 16) & 255, l = (q >> 24) & 255;
+    float g[4][3];
+    float th = dihedral_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, s.xyz + 3 * l, g[0],
+                             g[1], g[2], g[3]);
+    float dedt;
+    e += fourier(L.dih_per[li.dih_off + t], L.dih_kp + 3 * (li.dih_off + t), th, &dedt);
+    lig_grad_add(s, i, g[0], -dedt);
+    lig_grad_add(s, j, g[1], -dedt);
+    lig_grad_add(s, k, g[2], -dedt);
+    lig_grad_add(s, l, g[3], -dedt);
+  }
+  for (int p = lane; p < li.n_pairs; p += kWave) {
+    uint32_t w = L.pairs[li.pair_off + p];
+    int i = w & 255, j = (w >> 8) & 255;
+    bool is14 = (w >> 16) & 1;
+    float cs = is14 ? kScale14Coul : 1.f, vs = is14 ? kScale14Vdw : 1.f;
+    float r[3] = {s.xyz[3 * j] - s.xyz[3 * i], s.xyz[3 * j + 1] - s.xyz[3 * i + 1],
+                  s.xyz[3 * j + 2] - s.xyz[3 * i + 2]};
+    float2 la = s.lj[i], lb = s.lj[j];
+    float de;
+    e += pair_energy(r[0], r[1], r[2], s.q[i] * s.q[j] * cs, (la.x + lb.x) * 0.5f,
+                     la.y * lb.y * 4.f * vs, &de);
+    lig_grad_add(s, i, r, -de);
+    lig_grad_add(s, j, r, de);
+  }
+  return e;
+}
+
+// Trilinear value + stored-gradient interpolation (grid.cpp:273-303).
+__device__ inline float4 interp(const float4* map, const long long* off, const float* w) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float4 v = map[off[c]];
+    acc.x += v.x * w[c];
+    acc.y += v.y * w[c];
+    acc.z += v.z * w[c];
+    acc.w += v.w * w[c];
+  }
+  return acc;
+}
+
+// receptor_cache::get (grid.cpp:197-262) for one ligand atom: adds into e/g.
+__device__ inline void grid_get(const DevGrid& G, int slot, float q, const float* x,
+                                double* e, float* g) {
+  float f[3], gt[3], pen = 0.f;
+  int lo[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    f[d] = (x[d] - G.init[d]) * G.inv_spacing;
+    int nm1 = G.np[d] - 1;
+    if (f[d] < 0.f) {
+      lo[d] = 0;
+      gt[d] = -1.f;
+      pen -= f[d];
+      f[d] = 0.f;
+    } else if (f[d] < (float)nm1) {
+      lo[d] = (int)f[d];
+      gt[d] = 0.f;
+    } else {
+      lo[d] = nm1 - 1;
+      gt[d] = 1.f;
+      pen += f[d] - (float)nm1;
+      f[d] = (float)nm1;
+    }
+    f[d] -= (float)lo[d];
+  }
+  long long ny = G.np[1], nz = G.np[2];
+  long long base = ((long long)lo[0] * ny + lo[1]) * nz + lo[2];
+  long long off[8] = {base,
+                      base + ny * nz,
+                      base + nz,
+                      base + ny * nz + nz,
+                      base + 1,
+                      base + ny * nz + 1,
+                      base + nz + 1,
+                      base + ny * nz + nz + 1};
+  float x0 = f[0], y0 = f[1], z0 = f[2], mx = 1.f - x0, my = 1.f - y0, mz = 1.f - z0;
+  float w[8] = {mx * my * mz, x0 * my * mz, mx * y0 * mz, x0 * y0 * mz,
+                mx * my * z0, x0 * my * z0, mx * y0 * z0, x0 * y0 * z0};
+  bool hit = true;
+  if (pen > 0.f) {
+    *e += (double)(pen * G.spacing * G.slope);
+    g[0] += gt[0] * G.slope;
+    g[1] += gt[1] * G.slope;
+    g[2] += gt[2] * G.slope;
+    hit = false;
+  }
+  float4 c = interp(G.maps, off, w);
+  *e += (double)(c.x * q);
+  if (hit) {
+    g[0] += c.y * q;
+    g[1] += c.z * q;
+    g[2] += c.w * q;
+  }
+  float4 v = interp(G.maps + G.P * slot, off, w);
+  *e += (double)v.x;
+  if (hit) {
+    g[0] += v.y;
+    g[1] += v.z;
+    g[2] += v.w;
+  }
+}
+
+// inter_molecular_interactions::put_gradients (interaction.cpp:202-257).
+// With a grid: map lookup + pairs against movable receptor atoms; without:
+// pairs against every receptor atom (fixed atoms by broadcast loads).
+__device__ inline double inter(const DevReceptor& R, const DevGrid* G, const LigInfo& li,
+                               PoseLds& s) {
+  double e = 0.0;
+  const int lane = lane_id();
+  const int n = li.n_atoms;
+  // lane owns atoms lane and lane + 64
+  float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
+  const int a0 = lane, a1 = lane + kWave;
+  if (G) {
+    if (a0 < n) grid_get(*G, s.slot[a0], s.q[a0], s.xyz + 3 * a0, &e, g0);
+    if (a1 < n) grid_get(*G, s.slot[a1], s.q[a1], s.xyz + 3 * a1, &e, g1);
+  } else {
+    for (int k = 0; k < R.n_fixed; ++k) {
+      int j = R.fixed[k];
+      float4 pj = R.xyzq[j];
+      float2 lj = R.lj[j];
+      for (int h = 0; h < 2; ++h) {
+        int a = h ? a1 : a0;
+        if (a >= n) continue;
+        float* gg = h ? g1 : g0;
+        float rx = pj.x - s.xyz[3 * a], ry = pj.y - s.xyz[3 * a + 1], rz = pj.z - s.xyz[3 * a + 2];
+        float2 la = s.lj[a];
+        float de;
+        e += pair_energy(rx, ry, rz, s.q[a] * pj.w, (la.x + lj.x) * 0.5f, la.y * lj.y * 4.f, &de);
+        gg[0] -= rx * de;
+        gg[1] -= ry * de;
+        gg[2] -= rz * de;
+      }
+    }
+  }
+  // movable receptor atoms: explicit pairs, their gradient reduced per slot
+  for (int m = 0; m < R.n_mov; ++m) {
+    int atom = R.mov_atom[m];
+    float qm = R.xyzq[atom].w;
+    float2 lm = R.lj[atom];
+    float px = s.mov[3 * m], py = s.mov[3 * m + 1], pz = s.mov[3 * m + 2];
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    for (int h = 0; h < 2; ++h) {
+      int a = h ? a1 : a0;
+      if (a >= n) continue;
+      float* gg = h ? g1 : g0;
+      float rx = px - s.xyz[3 * a], ry = py - s.xyz[3 * a + 1], rz = pz - s.xyz[3 * a + 2];
+      float2 la = s.lj[a];
+      float de;
+      e += pair_energy(rx, ry, rz, s.q[a] * qm, (la.x + lm.x) * 0.5f, la.y * lm.y * 4.f, &de);
+      gg[0] -= rx * de;
+      gg[1] -= ry * de;
+      gg[2] -= rz * de;
+      gx += rx * de;
+      gy += ry * de;
+      gz += rz * de;
+    }
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    if (lane == 0) {
+      s.movg[3 * m] += gx;
+      s.movg[3 * m + 1] += gy;
+      s.movg[3 * m + 2] += gz;
+    }
+  }
+  if (a0 < n) lig_grad_add(s, a0, g0, 1.f);
+  if (a1 < n) lig_grad_add(s, a1, g1, 1.f);
+  return e;
+}
+
+// binding_system_interactions::put_gradients (interaction.cpp:271-298) for
+// the pose whose ligand coordinates and receptor torsions are in s.xyz /
+// s.tors.  Gradients are written to s.grad / s.torsg (overwrite).  Returns
+// the wave-uniform FP64 energy.  Uses __syncthreads(): one wave per block.
+__device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                                   const LigInfo& li, PoseLds& s) {
+  const int lane = lane_id();
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.grad[k] = 0.f;
+  receptor_torsion2xyz(R, s);
+  __syncthreads();
+  double e = receptor_intra(R, s);
+  if (lane == 0) e += R.e_const;
+  __syncthreads();
+  e += ligand_intra(L, li, s);
+  e += inter(R, G, li, s);
+  __syncthreads();
+  // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf
+  for (int k = lane; k < R.n_leaves; k += kWave) s.torsg[k] = 0.f;
+  __syncthreads();
+  for (int m = lane; m < R.n_mov; m += kWave) {
+    float v = s.jac[3 * m] * s.movg[3 * m] + s.jac[3 * m + 1] * s.movg[3 * m + 1] +
+              s.jac[3 * m + 2] * s.movg[3 * m + 2];
+    atomicAdd(&s.torsg[R.mov_leaf[m]], v);
+  }
+  __syncthreads();
+  return wave_sum(e);
+}
+
+}  // namespace pxd

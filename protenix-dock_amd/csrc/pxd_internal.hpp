@@ -1,0 +1,169 @@
+// pxd_internal.hpp — device-side data layout shared by the kernels and the
+// C-ABI implementation.  All device arrays are FP32/INT32 structure-of-arrays
+// resident in HBM; per-pose working sets live in LDS (one wavefront per pose).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pxdock.h"
+
+namespace pxd {
+
+constexpr int kWave = 64;
+constexpr int kMaxAtoms = PXD_MAX_LIGAND_ATOMS;  // per ligand (LDS budget)
+constexpr int kMaxFrags = PXD_MAX_FRAGMENTS;
+constexpr int kMaxLeaves = PXD_MAX_LEAVES;
+constexpr int kMaxMovable = 64;                  // movable receptor atoms
+constexpr int kMaxTypes = PXD_NUM_VDW_TYPES;
+constexpr float kCoulomb = 332.0637141491396f;   // system.h:83
+constexpr float kEps = 1e-20f;                   // math.h:128
+constexpr float kScale14Coul = 0.833333333f;     // interaction.h:52
+constexpr float kScale14Vdw = 0.5f;              // interaction.h:53
+
+// Receptor-term atom reference: >= 0 fixed receptor atom, < 0 movable slot (-ref-1).
+__host__ __device__ inline int mov_ref(int slot) { return -slot - 1; }
+
+struct DevReceptor {
+  int n_atoms;
+  const float4* xyzq;   // [n] initial x, y, z, charge
+  const float2* lj;     // [n] sigma, sqrt(epsilon)
+  int n_fixed;          // atoms that are not movable (grid / brute-force inter)
+  const int* fixed;     // [n_fixed]
+  // leaves (data.cpp:348-363)
+  int n_leaves;
+  int n_mov;            // movable slots
+  const float4* leaf_origin;  // [L]
+  const float4* leaf_axis;    // [L]
+  const int* mov_leaf;        // [M] leaf of each slot
+  const int* mov_atom;        // [M] receptor atom of each slot
+  // receptor intra terms that touch a movable atom (receptor.py:230-265)
+  int n_bonds;
+  const int2* bond_ab;        // refs
+  const float2* bond_kr;
+  int n_angles;
+  const int4* angle_abc;      // refs (x, y, z)
+  const float2* angle_kt;
+  int n_dih;
+  const int4* dih_abcd;       // refs
+  const uint32_t* dih_per;    // 6 x 4-bit periodicities
+  const float4* dih_kp;       // [n][3]: (k0,k1,k2,k3),(k4,k5,p0,p1),(p2,p3,p4,p5)
+  // movable-vs-fixed non-bonded pairs, segmented per slot, each segment padded
+  // to a multiple of 64 with inert entries
+  const int* rp_seg;          // [M+1] offsets
+  const float4* rp_pos;       // partner x, y, z, q * coul_scale
+  const float2* rp_lj;        // partner sigma, sqrt(eps) * 4 * lj_scale
+  // movable-vs-movable pairs (x = slot a, y = slot b, z = is14)
+  int n_mm;
+  const int4* mm;
+  // pscore interaction sets (receptor side)
+  int n_hydro, n_cation, n_anion, n_ring5, n_ring6;
+  int n_don_c, n_don_n, n_acc_c, n_acc_n;
+  const int* hydro;
+  const int* cation;
+  const int* anion;
+  const int* ring5;           // [n][5]
+  const int* ring6;           // [n][6]
+  const int2* don_c;
+  const int2* don_n;
+  const int* acc_c;
+  const int* acc_n;
+  const int* atom_mov_slot;   // [n] slot or -1
+  // fixed atoms compacted for the grid build
+  const float4* fixed_xyzq;
+  const float2* fixed_lj;
+  // FP64 energy of receptor terms touching no movable atom (pose independent)
+  double e_const;
+};
+
+// Per-ligand record of the packed batch.
+struct LigInfo {
+  int n_atoms, atom_off;
+  int bond_off, n_bonds;
+  int angle_off, n_angles;
+  int dih_off, n_dih;
+  int pair_off, n_pairs;
+  int frag_off, n_frags, n_tors;
+  int hydro_off, n_hydro;
+  int cation_off, n_cation;
+  int anion_off, n_anion;
+  int ring5_off, n_ring5;
+  int ring6_off, n_ring6;
+  int donc_off, n_donc;
+  int donn_off, n_donn;
+  int accc_off, n_accc;
+  int accn_off, n_accn;
+  int ts_off, n_ts;
+  int n_rot;
+  int tsm_off;   // first entry of this ligand in ts_moff (n_ts + 1 entries)
+};
+
+struct DevLigands {
+  int n;
+  const LigInfo* info;
+  const int* type;        // per atom GAFF2 index
+  const float* q;         // per atom charge
+  const float2* lj;       // per atom sigma, sqrt(epsilon) (GAFF2 table)
+  const uint32_t* bond_ij;  // i | j << 8
+  const float2* bond_kr;
+  const uint32_t* angle_ijk;  // i | j << 8 | k << 16
+  const float2* angle_kt;
+  const uint32_t* dih_ijkl;   // i | j << 8 | k << 16 | l << 24
+  const uint32_t* dih_per;
+  const float4* dih_kp;       // [n][3] as DevReceptor
+  const uint32_t* pairs;      // i | j << 8 | is14 << 16
+  // fragment trees (permuted-slot form of pxd_frag_tree)
+  const int* frag;            // [frag_off*6]: start, end, parent, bond_parent, bond_mine, pad
+  const int* tors_frag;       // [frag_off]  (per ligand n_tors entries)
+  const int* perm;            // [atom_off] permuted slot -> atom
+  // pscore sets (ligand-local atom indices)
+  const int* sets;            // hydro/cation/anion/acc lists packed
+  const int* rings;           // ring5 (5 ints) / ring6 (6 ints) packed
+  const int2* dons;           // donor pairs
+  const float4* ts_coef;      // torsion-strain coefs
+  const int* ts_moff;         // per ligand n_ts+1 absolute offsets into ts_ijkl
+  const uint32_t* ts_ijkl;    // packed atoms
+};
+
+struct DevGrid {
+  float init[3];
+  int np[3];
+  float spacing, inv_spacing;
+  float slope;
+  long long P;                // points per map
+  const float4* maps;         // [(n_types+1)][P], map 0 = Coulomb
+  int type_slot[kMaxTypes];   // GAFF2 index -> map index (-1: none)
+};
+
+}  // namespace pxd
+
+// Opaque handle definitions (C-ABI side).
+struct pxd_receptor {
+  pxd::DevReceptor dev;
+  int n_atoms;
+  void* mem;                  // single device allocation backing dev arrays
+  size_t mem_bytes;
+  // host copies needed by other entry points
+  float* h_xyz;               // initial coordinates
+  int n_leaves;
+  int n_mov;
+};
+
+struct pxd_grid {
+  pxd::DevGrid dev;
+  pxd_box box;
+  int n_types;
+  int32_t types[PXD_NUM_VDW_TYPES];
+  void* mem;
+  const pxd_receptor* receptor;
+};
+
+struct pxd_ligand_batch {
+  pxd::DevLigands dev;
+  void* mem;
+  size_t mem_bytes;
+  int n;
+  int* h_natoms;
+  int* h_ntors;
+  uint8_t type_used[PXD_NUM_VDW_TYPES];
+};

@@ -1,0 +1,628 @@
+// pxd_kernels.hip — grid build, batched score+grad and persistent batched
+// L-BFGS kernels for gfx950, with their host launchers.
+#include <hip/hip_runtime.h>
+
+#include "pxd_device.hpp"
+#include "pxd_kernels.hpp"
+#include "pxd_pscore.hpp"
+
+namespace pxd {
+
+// Blocks are handed to the 8 XCDs round-robin; give each XCD a contiguous
+// range of logical blocks so consecutive poses (same ligand, same term
+// lists) share one XCD's L2.
+__device__ inline int xcd_block(int b, int nb) {
+  const int nx = 8;
+  int q = nb / nx, r = nb % nx, x = b % nx, i = b / nx;
+  return x * q + (x < r ? x : r) + i;
+}
+
+// ============================================================== K1: grid build
+// receptor_cache::partial_worker::populate (grid.cpp:150-195): for every grid
+// point, Coulomb for a unit probe and LJ per ligand type, each with the probe
+// gradient, summed over the fixed (non-movable) receptor atoms.  One thread
+// per point, TG types per pass, receptor atoms staged through LDS tiles.
+
+template <int TG>
+__global__ __launch_bounds__(256) void k_grid(const float4* __restrict__ atoms,
+                                             const float2* __restrict__ alj, int n_atoms,
+                                             const float2* __restrict__ tlj, int n_types,
+                                             int type0, float ix, float iy, float iz, int ny,
+                                             int nz, float spacing, long long P,
+                                             float4* __restrict__ maps, int with_coulomb) {
+  constexpr int kTile = 256;
+  __shared__ float4 s_at[kTile];
+  __shared__ float2 s_lj[kTile];
+  __shared__ float2 s_tlj[TG];
+  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int nt = n_types - type0 < TG ? n_types - type0 : TG;
+  if (threadIdx.x < TG) s_tlj[threadIdx.x] = threadIdx.x < nt ? tlj[type0 + threadIdx.x] : make_float2(0.f, 0.f);
+  int k = (int)(p % nz);
+  int j = (int)((p / nz) % ny);
+  int i = (int)(p / ((long long)nz * ny));
+  // grid.cpp:165-169 (init + spacing * index)
+  float px = ix + spacing * i, py = iy + spacing * j, pz = iz + spacing * k;
+  double ce = 0.0, cg0 = 0.0, cg1 = 0.0, cg2 = 0.0;
+  float acc[TG][4];
+#pragma unroll
+  for (int t = 0; t < TG; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
+  for (int base = 0; base < n_atoms; base += kTile) {
+    __syncthreads();
+    int a = base + threadIdx.x;
+    if (a < n_atoms) {
+      s_at[threadIdx.x] = atoms[a];
+      s_lj[threadIdx.x] = alj[a];
+    }
+    __syncthreads();
+    int cnt = n_atoms - base < kTile ? n_atoms - base : kTile;
+    for (int u = 0; u < cnt; ++u) {
+      float4 at = s_at[u];
+      float2 lj = s_lj[u];
+      float vx = at.x - px, vy = at.y - py, vz = at.z - pz;
+      float d2 = vx * vx + vy * vy + vz * vz;
+      float d = sqrtf(d2);
+      float inv_d = 1.f / (d + kEps);
+      float irsqr = 1.f / (d2 + kEps);
+      if (with_coulomb) {
+        float e = at.w * inv_d * kCoulomb;
+        float de = -(at.w / (d2 * d + kEps)) * kCoulomb;
+        ce += e;
+        cg0 -= (double)(vx * de);
+        cg1 -= (double)(vy * de);
+        cg2 -= (double)(vz * de);
+      }
+#pragma unroll
+      for (int t = 0; t < TG; ++t) {
+        float2 tl = s_tlj[t];
+        float uu = (tl.x + lj.x) * 0.5f * inv_d;
+        float u2 = uu * uu;
+        float u6 = u2 * u2 * u2;
+        float u12 = u6 * u6;
+        float e4 = tl.y * lj.y;
+        acc[t][0] += (u12 - u6) * e4;
+        float de = (-12.f * u12 + 6.f * u6) * irsqr * e4;
+        acc[t][1] -= vx * de;
+        acc[t][2] -= vy * de;
+        acc[t][3] -= vz * de;
+      }
+    }
+  }
+  if (p >= P) return;
+  const float big = 3.0e37f;  // FP32 maps saturate instead of overflowing
+  auto sat = [&](float v) { return fminf(fmaxf(v, -big), big); };
+  if (with_coulomb)
+    maps[p] = make_float4(sat((float)ce), sat((float)cg0), sat((float)cg1), sat((float)cg2));
+#pragma unroll
+  for (int t = 0; t < TG; ++t)
+    if (t < nt)
+      maps[(long long)(1 + type0 + t) * P + p] =
+          make_float4(sat(acc[t][0]), sat(acc[t][1]), sat(acc[t][2]), sat(acc[t][3]));
+}
+
+void launch_grid_build(const GridBuildArgs& a, hipStream_t st) {
+  const int threads = 256;
+  long long blocks = (a.P + threads - 1) / threads;
+  for (int t0 = 0, first = 1; t0 < a.n_types || first; t0 += 8, first = 0) {
+    hipLaunchKernelGGL(k_grid<8>, dim3((unsigned)blocks), dim3(threads), 0, st, a.atoms, a.alj,
+                       a.n_atoms, a.tlj, a.n_types, t0, a.init[0], a.init[1], a.init[2], a.np[1],
+                       a.np[2], a.spacing, a.P, a.maps, first);
+  }
+}
+
+// ============================================================== K2: score+grad
+// One wavefront (= one block) per pose.
+
+__global__ __launch_bounds__(64) void k_score(DevReceptor R, DevLigands L,
+                                              const DevGrid* __restrict__ G, int n_poses,
+                                              const int* __restrict__ pose_lig,
+                                              const int* __restrict__ pose_off,
+                                              const float* __restrict__ xyz,
+                                              const float* __restrict__ tors,
+                                              double* __restrict__ energy,
+                                              float* __restrict__ lig_grad,
+                                              float* __restrict__ tors_grad) {
+  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  PoseLds s = carve_pose_lds(lds);
+  const int p = xcd_block(blockIdx.x, gridDim.x);
+  if (p >= n_poses) return;
+  const int lane = lane_id();
+  const LigInfo li = L.info[pose_lig[p]];
+  load_ligand_params(L, li, G, s);
+  const long long off = 3ll * pose_off[p];
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.xyz[k] = xyz[off + k];
+  for (int k = lane; k < R.n_leaves; k += kWave)
+    s.tors[k] = tors ? tors[(long long)p * R.n_leaves + k] : 0.f;
+  __syncthreads();
+  double e = eval_pose(R, L, G, li, s);
+  if (lane == 0) energy[p] = e;
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) lig_grad[off + k] = s.grad[k];
+  if (tors_grad)
+    for (int k = lane; k < R.n_leaves; k += kWave)
+      tors_grad[(long long)p * R.n_leaves + k] = s.torsg[k];
+}
+
+void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                  const ScoreArgs& a, hipStream_t st) {
+  if (a.n_poses <= 0) return;
+  hipLaunchKernelGGL(k_score, dim3(a.n_poses), dim3(kWave), 0, st, R, L, G, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.energy, a.lig_grad, a.tors_grad);
+}
+
+// ============================================================== K3: L-BFGS
+// lbfgs_step::apply + strong_wolfe (optimizer.cpp:205-560), one pose per
+// wavefront, many poses per launch from a device work queue.  The flat
+// variable [ligand xyz | receptor torsions] is distributed over the lanes
+// (element e = lane + 64 k); each lane only ever touches its own elements of
+// the LDS-resident vectors, so vector algebra needs no synchronisation.  The
+// s/y history lives in the slot's global scratch; scalar control (f, t, gtd,
+// curvature s.y / y.y, Armijo) is FP64.
+
+// Per-wave LDS vector pool.
+enum { V_X = 0, V_G, V_GP, V_D, V_W0, V_W1, V_W2, V_W3, V_COUNT };
+
+template <int VPL>
+struct LbfgsCtx {
+  const DevReceptor* R;
+  const DevLigands* L;
+  const DevGrid* G;
+  LigInfo li;
+  PoseLds s;
+  float* v;  // V_COUNT vectors of kWave*VPL floats
+  int n3, n;
+  int nevals;
+  __device__ float* vec(int i) const { return v + i * (kWave * VPL); }
+};
+
+template <int VPL>
+__device__ inline double vdot(const float* a, const float* b) {
+  const int lane = lane_id();
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) acc += (double)a[lane + kWave * k] * (double)b[lane + kWave * k];
+  return wave_sum(acc);
+}
+
+template <int VPL>
+__device__ inline float vamax(const float* a) {
+  const int lane = lane_id();
+  float m = 0.f;
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) m = fmaxf(m, fabsf(a[lane + kWave * k]));
+  return wave_max(m);
+}
+
+template <int VPL>
+__device__ inline void vcopy(float* dst, const float* src) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) dst[lane + kWave * k] = src[lane + kWave * k];
+}
+
+// Two-element bracket state held in scalars (a runtime-indexed local array
+// would live in scratch memory).
+struct Pair2 {
+  double a, b;
+  __device__ double get(int i) const { return i ? b : a; }
+  __device__ void set(int i, double v) {
+    if (i)
+      b = v;
+    else
+      a = v;
+  }
+};
+struct IPair2 {
+  int a, b;
+  __device__ int get(int i) const { return i ? b : a; }
+  __device__ void set(int i, int v) {
+    if (i)
+      b = v;
+    else
+      a = v;
+  }
+};
+
+// obj_func_closure::evaluate_direction (optimizer.cpp:155-167): f and g at x + t d.
+template <int VPL>
+__device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, float* g) {
+  const int lane = lane_id();
+  const float* x = c.vec(V_X);
+  float tf = (float)t;
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    int e = lane + kWave * k;
+    float xv = x[e] + d[e] * tf;
+    if (e < c.n3)
+      c.s.xyz[e] = xv;
+    else if (e < c.n)
+      c.s.tors[e - c.n3] = xv;
+  }
+  __syncthreads();
+  double f = eval_pose(*c.R, *c.L, c.G, c.li, c.s);
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) {
+    int e = lane + kWave * k;
+    g[e] = e < c.n3 ? c.s.grad[e] : (e < c.n ? c.s.torsg[e - c.n3] : 0.f);
+  }
+  __syncthreads();
+  c.nevals++;
+  return f;
+}
+
+// optimizer.cpp:205-233
+__device__ inline double cubic_interpolate(double x1, double f1, double g1, double x2, double f2,
+                                           double g2, double lo, double hi) {
+  double d1 = g1 + g2 - 3.0 * (f1 - f2) / (x1 - x2);
+  double d2sq = d1 * d1 - g1 * g2;
+  if (d2sq >= 0.0) {
+    double d2 = sqrt(d2sq), mp;
+    if (x1 <= x2)
+      mp = x2 - (x2 - x1) * ((g2 + d2 - d1) / (g2 - g1 + 2.0 * d2));
+    else
+      mp = x1 - (x1 - x2) * ((g1 + d2 - d1) / (g1 - g2 + 2.0 * d2));
+    double t = mp > lo ? mp : lo;
+    return t < hi ? t : hi;
+  }
+  return (lo + hi) / 2.0;
+}
+
+// strong_wolfe (optimizer.cpp:235-427).  Gradient vectors are named buffers
+// in the LDS pool; the reference's std::move of vectors becomes a swap of
+// buffer indices (uniform scalars), so no vector is copied in the loop.
+template <int VPL>
+__device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* t_io,
+                             const float* d, double* f_io, double gtd) {
+  double t = *t_io, f = *f_io;
+  int b_new = V_W0, b_prev = V_W1;
+  Pair2 br{0, 0}, bf{0, 0}, bgtd{0, 0};
+  IPair2 idx{V_W2, V_W3};  // buffers of bracket_g[0], bracket_g[1]
+  double f_new = eval_dir(c, t, d, c.vec(b_new));
+  if (isnan(f_new)) {
+    *f_io = f_new;
+    return;
+  }
+  double gtd_new = vdot<VPL>(c.vec(b_new), d);
+  if (fabs(gtd_new) > p.gtd_threshold) {
+    *f_io = f_new;
+    vcopy<VPL>(c.vec(V_G), c.vec(b_new));
+    return;
+  }
+  double t_prev = 0.0, f_prev = f, gtd_prev = gtd;
+  vcopy<VPL>(c.vec(b_prev), c.vec(V_G));
+  bool done = false;
+  int ls_iter = 0;
+  while (ls_iter < p.max_ls) {
+    if ((f_new > f + p.c1 * t * gtd) || (ls_iter > 1 && f_new >= f_prev)) {
+      br = {t_prev, t}; bf = {f_prev, f_new}; bgtd = {gtd_prev, gtd_new};
+      idx = {b_prev, b_new};
+      break;
+    }
+    if (fabs(gtd_new) <= -p.c2 * gtd) {
+      br.a = t; bf.a = f_new;
+      idx = {b_new, b_prev};
+      done = true;
+      break;
+    }
+    if (gtd_new >= 0) {
+      br = {t_prev, t}; bf = {f_prev, f_new}; bgtd = {gtd_prev, gtd_new};
+      idx = {b_prev, b_new};
+      break;
+    }
+    double lo = t + 0.01 * (t - t_prev), hi = t * 10.0, tmp = t;
+    t = cubic_interpolate(t_prev, f_prev, gtd_prev, t, f_new, gtd_new, lo, hi);
+    t_prev = tmp;
+    f_prev = f_new;
+    int sw = b_prev;  // g_prev <- g_new (a move): swap buffer roles
+    b_prev = b_new;
+    b_new = sw;
+    gtd_prev = gtd_new;
+    f_new = eval_dir(c, t, d, c.vec(b_new));
+    gtd_new = vdot<VPL>(c.vec(b_new), d);
+    ls_iter++;
+  }
+  double d_norm = 0.0;
+  if (ls_iter == p.max_ls) {
+    br = {0.0, t}; bf = {f, f_new};
+    vcopy<VPL>(c.vec(b_prev), c.vec(V_G));  // bracket_g[0] = g
+    idx = {b_prev, b_new};
+  } else {
+    d_norm = (double)vamax<VPL>(d);
+  }
+  // a pool buffer outside the bracket receives each zoom evaluation
+  int z_new = V_W0;
+  while (z_new == idx.a || z_new == idx.b) ++z_new;
+  bool insuf = false;
+  int low = 0, high = 1;
+  if (!done && bf.a > bf.b) {
+    low = 1;
+    high = 0;
+  }
+  while (!done && ls_iter < p.max_ls) {
+    double lo, hi;
+    if (br.a > br.b) { lo = br.b; hi = br.a; } else { lo = br.a; hi = br.b; }
+    double eps = hi - lo;
+    if (eps * d_norm < p.ls_tol_change) break;
+    t = cubic_interpolate(br.a, bf.a, bgtd.a, br.b, bf.b, bgtd.b, lo, hi);
+    eps *= 0.1;
+    double m1 = hi - t, m2 = t - lo;
+    if ((m1 < m2 ? m1 : m2) < eps) {
+      if (insuf || t >= hi || t <= lo) {
+        t = fabs(t - hi) < fabs(t - lo) ? hi - eps : lo + eps;
+        insuf = false;
+      } else {
+        insuf = true;
+      }
+    } else {
+      insuf = false;
+    }
+    f_new = eval_dir(c, t, d, c.vec(z_new));
+    gtd_new = vdot<VPL>(c.vec(z_new), d);
+    ls_iter++;
+    if (f_new > (f + p.c1 * t * gtd) || f_new >= bf.get(low)) {
+      // bracket[high] <- new point; its old buffer becomes the zoom scratch
+      br.set(high, t); bf.set(high, f_new); bgtd.set(high, gtd_new);
+      int old = idx.get(high);
+      idx.set(high, z_new);
+      z_new = old;
+      if (bf.a > bf.b) { low = 1; high = 0; } else { low = 0; high = 1; }
+    } else {
+      if (fabs(gtd_new) <= -p.c2 * gtd) {
+        done = true;
+      } else if (gtd_new * (br.get(high) - br.get(low)) >= 0.0) {
+        // old high becomes new low
+        br.set(high, br.get(low)); bf.set(high, bf.get(low)); bgtd.set(high, bgtd.get(low));
+        int old = idx.get(high);
+        idx.set(high, idx.get(low));
+        idx.set(low, old);
+      }
+      br.set(low, t); bf.set(low, f_new); bgtd.set(low, gtd_new);
+      int old = idx.get(low);
+      idx.set(low, z_new);
+      z_new = old;
+    }
+  }
+  *t_io = br.get(low);
+  *f_io = bf.get(low);
+  vcopy<VPL>(c.vec(V_G), c.vec(idx.get(low)));
+}
+
+template <int VPL>
+__global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
+                                              const DevGrid* __restrict__ G, LbfgsArgs A) {
+  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ float vpool[V_COUNT * kWave * VPL];
+  __shared__ double s_ro[kMaxHistory];
+  __shared__ double s_al[kMaxHistory];
+  __shared__ int s_pose;
+  const int lane = lane_id();
+  const LbfgsParamsDev& P = A.prm;
+  const int m = P.history;
+  constexpr int NV = kWave * VPL;
+  float* hy = A.hist + (size_t)blockIdx.x * 2 * m * NV;
+  float* hs = hy + (size_t)m * NV;
+  LbfgsCtx<VPL> c;
+  c.R = &R;
+  c.L = &L;
+  c.G = G;
+  c.s = carve_pose_lds(lds);
+  c.v = vpool;
+  float* x = c.vec(V_X);
+  float* g = c.vec(V_G);
+  float* gp = c.vec(V_GP);
+  float* d = c.vec(V_D);
+  float* y = c.vec(V_W0);
+  float* sv = c.vec(V_W1);
+  float* q = c.vec(V_W2);
+  for (;;) {
+    if (lane == 0) s_pose = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int pi = s_pose;
+    __syncthreads();
+    if (pi >= A.n_poses) break;
+    c.li = L.info[A.pose_lig[pi]];
+    load_ligand_params(L, c.li, G, c.s);
+    c.n3 = 3 * c.li.n_atoms;
+    c.n = c.n3 + R.n_leaves;
+    c.nevals = 0;
+    const long long off = 3ll * A.pose_off[pi];
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+      int e = lane + kWave * k;
+      x[e] = e < c.n3 ? A.xyz[off + e]
+                      : (e < c.n ? A.tors[(long long)pi * R.n_leaves + (e - c.n3)] : 0.f);
+      d[e] = 0.f;
+    }
+    // initial f(x), df/dx (optimizer.cpp:435-441)
+    double loss = eval_dir(c, 0.0, d, g);
+    double energy = loss;  // kept unless a step is taken (optimizer.cpp:554-558)
+    bool converged = false, updated = false;
+    if (vamax<VPL>(g) < (float)P.tol_grad) {
+      converged = true;
+    } else {
+      int head = 0, num_old = 0, n_iter = 0;
+      bool high_change = false;
+      double H = 1.0, t = 0.0, prev_loss;
+      while (n_iter < P.max_iter && c.nevals < P.max_eval) {
+        n_iter++;
+        if (n_iter > 1) {
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) y[lane + kWave * k] = g[lane + kWave * k] - gp[lane + kWave * k];
+          high_change = vamax<VPL>(y) > 1e8f;
+        }
+        if (n_iter == 1 || high_change) {
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = -g[lane + kWave * k];
+          num_old = 0;
+          head = 0;
+          H = 1.0;
+        } else {
+          float tf = (float)t;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) sv[lane + kWave * k] = d[lane + kWave * k] * tf;
+          double ys = vdot<VPL>(y, sv);
+          if (ys > 1e-10) {  // optimizer.cpp:473-487
+            int slot;
+            if (num_old == m) {
+              slot = head;
+              head = (head + 1) % m;
+            } else {
+              slot = (head + num_old) % m;
+              num_old++;
+            }
+            H = ys / vdot<VPL>(y, y);
+#pragma unroll
+            for (int k = 0; k < VPL; ++k) {
+              hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
+              hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
+            }
+            if (lane == 0) s_ro[slot] = 1.0 / ys;
+            __syncthreads();
+          }
+          // two-loop recursion (optimizer.cpp:493-507)
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) q[lane + kWave * k] = -g[lane + kWave * k];
+          for (int h = num_old - 1; h >= 0; --h) {
+            int slot = (head + h) % m;
+            const float* sh = hs + (size_t)slot * NV;
+            const float* yh = hy + (size_t)slot * NV;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < VPL; ++k)
+              acc += (double)sh[lane + kWave * k] * (double)q[lane + kWave * k];
+            double al = wave_sum(acc) * s_ro[slot];
+            if (lane == 0) s_al[h] = al;
+            float alf = (float)al;
+#pragma unroll
+            for (int k = 0; k < VPL; ++k) q[lane + kWave * k] -= alf * yh[lane + kWave * k];
+          }
+          __syncthreads();
+          float Hf = (float)H;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = q[lane + kWave * k] * Hf;
+          for (int h = 0; h < num_old; ++h) {
+            int slot = (head + h) % m;
+            const float* sh = hs + (size_t)slot * NV;
+            const float* yh = hy + (size_t)slot * NV;
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < VPL; ++k)
+              acc += (double)yh[lane + kWave * k] * (double)d[lane + kWave * k];
+            double be = wave_sum(acc) * s_ro[slot];
+            float cf = (float)(s_al[h] - be);
+#pragma unroll
+            for (int k = 0; k < VPL; ++k) d[lane + kWave * k] += sh[lane + kWave * k] * cf;
+          }
+        }
+        vcopy<VPL>(gp, g);
+        prev_loss = loss;
+        if (n_iter == 1 || high_change) {
+          double l1 = 0.0;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) l1 += fabs((double)g[lane + kWave * k]);
+          l1 = wave_sum(l1);
+          double inv = 1.0 / l1;
+          t = (1.0 < inv ? 1.0 : inv) * P.lr * 0.1;
+        } else {
+          t = P.lr;
+        }
+        double gtd = vdot<VPL>(g, d);
+        if (gtd > -P.tol_change) {
+          converged = true;
+          break;
+        }
+        strong_wolfe<VPL>(c, P, &t, d, &loss, gtd);
+        if (isnan(loss)) break;
+        float tf = (float)t;
+        float smax = 0.f;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+          x[lane + kWave * k] += d[lane + kWave * k] * tf;
+          smax = fmaxf(smax, fabsf(d[lane + kWave * k] * tf));
+        }
+        updated = true;
+        if (vamax<VPL>(g) <= (float)P.tol_grad) {
+          converged = true;
+          break;
+        }
+        if (wave_max(smax) <= (float)P.tol_change) {
+          converged = true;
+          break;
+        }
+        if (fabs(loss - prev_loss) < P.tol_change) {
+          converged = true;
+          break;
+        }
+      }
+      // optimizer.cpp:553-558: one final evaluation only after a step
+      if (updated) {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
+        energy = eval_dir(c, 0.0, d, g);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+      int e = lane + kWave * k;
+      if (e < c.n3)
+        A.xyz[off + e] = x[e];
+      else if (e < c.n)
+        A.tors[(long long)pi * R.n_leaves + (e - c.n3)] = x[e];
+    }
+    if (lane == 0) {
+      A.energy[pi] = energy;
+      A.nevals[pi] = c.nevals;
+      A.converged[pi] = converged ? 1 : 0;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                  const LbfgsArgs& a, int n_slots, hipStream_t st) {
+  hipLaunchKernelGGL(k_lbfgs<kLbfgsVpl>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+}
+
+// ============================================================== K5: pscore
+// root_scorer::combine for a batch of poses, one wavefront per pose.
+
+__global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_score_params sp,
+                                               int n_poses, const int* __restrict__ pose_lig,
+                                               const int* __restrict__ pose_off,
+                                               const float* __restrict__ xyz,
+                                               const float* __restrict__ tors,
+                                               double* __restrict__ score,
+                                               double* __restrict__ terms) {
+  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
+  __shared__ double s_terms[PXD_NUM_TERMS];
+  PoseLds s = carve_pose_lds(lds);
+  PscoreLds ps;
+  ps.rc = rings;
+  ps.rn = rings + 3 * kMaxRecRings;
+  ps.lc = rings + 6 * kMaxRecRings;
+  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps.terms = s_terms;
+  const int p = xcd_block(blockIdx.x, gridDim.x);
+  if (p >= n_poses) return;
+  const int lane = lane_id();
+  const LigInfo li = L.info[pose_lig[p]];
+  load_ligand_params(L, li, nullptr, s);
+  const long long off = 3ll * pose_off[p];
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.xyz[k] = xyz[off + k];
+  for (int k = lane; k < R.n_leaves; k += kWave)
+    s.tors[k] = tors ? tors[(long long)p * R.n_leaves + k] : 0.f;
+  __syncthreads();
+  receptor_torsion2xyz(R, s);
+  __syncthreads();
+  double v = pscore_pose(R, L, li, sp, s, ps);
+  if (lane == 0) score[p] = v;
+  if (terms && lane < PXD_NUM_TERMS) terms[(long long)p * PXD_NUM_TERMS + lane] = s_terms[lane];
+}
+
+void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
+                   const PscoreArgs& a, hipStream_t st) {
+  if (a.n_poses <= 0) return;
+  hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms);
+}
+
+}  // namespace pxd

@@ -476,7 +476,7 @@ def receptor_json(m):
 
 def ligand_json(m):
     t = m.tree
-    n = m.n_atoms
+    n = int(np.asarray(m.xyz).shape[-2])
     levels = []
     children = {f: [] for f in range(t.n_frags)}
     for f in range(t.n_frags):

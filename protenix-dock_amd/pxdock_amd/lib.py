@@ -1,0 +1,270 @@
+"""ctypes binding of libpxdock.so, the C ABI of include/pxdock.h.
+
+This is the product path: every call goes to the HIP library.  There is no
+CPU fallback: loading fails loudly when the library is missing.  Device
+buffers are torch tensors on the current HIP device (torch is plumbing for
+memory and streams only).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import ffi
+from .ffi import f32p, f64p, i32p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpxdock.so")
+
+_lib = None
+
+
+class PxdError(RuntimeError):
+    """A pxd_status other than PXD_OK, with pxd_last_error() text."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"pxd status {status}: {msg}")
+        self.status = status
+
+
+def load():
+    """Load libpxdock.so (raise if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    # torch ships its own HIP runtime; loading it first makes libpxdock.so bind
+    # to that same runtime (one HIP/HSA runtime per process).
+    import torch  # noqa: F401
+    L = C.CDLL(LIB_PATH)
+    P, i, d = C.c_void_p, C.c_int32, C.c_double
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "pxd_last_error": (C.c_char_p, []),
+        "pxd_version": (C.c_char_p, []),
+        "pxd_device_count": (i, [i32p]),
+        "pxd_lbfgs_default_params": (None, [P, i]),
+        "pxd_describe_boundary": (None, [f64p, f64p, P]),
+        "pxd_decide_grid_dims": (None, [f64p, f64p, d, P]),
+        "pxd_receptor_create": (i, [P, f64p, P, PP]),
+        "pxd_receptor_destroy": (None, [P]),
+        "pxd_grid_build": (i, [P, P, i32p, i, d, P, PP]),
+        "pxd_grid_upload": (i, [P, P, i32p, i, d, f32p, PP]),
+        "pxd_grid_download": (i, [P, f32p, C.c_size_t]),
+        "pxd_grid_info": (i, [P, P, i32p]),
+        "pxd_grid_set_slope": (i, [P, d]),
+        "pxd_grid_destroy": (None, [P]),
+        "pxd_ligand_batch_create": (i, [P, P, i, PP]),
+        "pxd_ligand_batch_destroy": (None, [P]),
+        "pxd_ligand_batch_natoms": (i, [P, i, i32p]),
+        "pxd_score_grad_batch": (i, [P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_lbfgs_batch": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_pscore_batch": (i, [P, P, P, i, P, P, P, P, P, P, P]),
+        "pxd_lbfgs_plain": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != 0:
+        raise PxdError(status, load().pxd_last_error().decode())
+
+
+def _ref(s):
+    return C.cast(C.pointer(s), C.c_void_p)
+
+
+def _dev(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def decide_grid_dims(center, size, spacing):
+    """decide_grid_dims (engine/src/core/grid.cpp:65-86)."""
+    b = ffi.PxdBox()
+    load().pxd_decide_grid_dims(ffi.as_f64(center).ctypes.data_as(f64p),
+                                ffi.as_f64(size).ctypes.data_as(f64p), spacing, _ref(b))
+    return b
+
+
+def describe_boundary(center, size):
+    b = ffi.PxdBox()
+    load().pxd_describe_boundary(ffi.as_f64(center).ctypes.data_as(f64p),
+                                 ffi.as_f64(size).ctypes.data_as(f64p), _ref(b))
+    return b
+
+
+class Receptor:
+    """Device-resident torsional_receptor (engine/include/bytedock/core/data.h:34-71)."""
+
+    def __init__(self, mol):
+        self.mol = mol
+        self._c = mol.to_c()
+        self._lv = mol.leaves.to_c() if mol.leaves is not None else None
+        self._xyz = ffi.as_f64(mol.xyz)
+        h = C.c_void_p()
+        check(load().pxd_receptor_create(_ref(self._c), self._xyz.ctypes.data_as(f64p),
+                                         _ref(self._lv) if self._lv is not None else None,
+                                         C.byref(h)))
+        self.h = h
+        self.n_leaves = mol.leaves.n_leaves if mol.leaves is not None else 0
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pxd_receptor_destroy(self.h)
+            self.h = None
+
+
+class Grid:
+    """Device-resident receptor_cache maps (grid.h:60-133), FP32 value+gradient."""
+
+    def __init__(self, receptor, box, types, slope=1e6, maps=None):
+        self.receptor = receptor
+        self.box = box
+        self.types = np.ascontiguousarray(sorted(set(int(t) for t in types)), dtype=np.int32)
+        h = C.c_void_p()
+        if maps is None:
+            check(load().pxd_grid_build(receptor.h, _ref(box), self.types.ctypes.data_as(i32p),
+                                        len(self.types), slope, _stream(), C.byref(h)))
+        else:
+            m = np.ascontiguousarray(maps, dtype=np.float32)
+            check(load().pxd_grid_upload(receptor.h, _ref(box), self.types.ctypes.data_as(i32p),
+                                         len(self.types), slope, m.ctypes.data_as(f32p),
+                                         C.byref(h)))
+        self.h = h
+
+    def download(self):
+        n = (len(self.types) + 1) * self.box.n_points() * 4
+        out = np.empty(n, np.float32)
+        check(load().pxd_grid_download(self.h, out.ctypes.data_as(f32p), n))
+        return out.reshape(len(self.types) + 1, self.box.n_points(), 4)
+
+    def set_slope(self, slope):
+        check(load().pxd_grid_set_slope(self.h, slope))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pxd_grid_destroy(self.h)
+            self.h = None
+
+
+class LigandBatch:
+    """Packed uneven ligands (free_ligand, data.h:73-99) on the device."""
+
+    def __init__(self, ligands):
+        self.ligands = list(ligands)
+        self._c = [m.to_c() for m in self.ligands]
+        mols = (ffi.PxdMol * len(self._c))(*self._c)
+        trees = None
+        if all(m.tree is not None for m in self.ligands):
+            self._t = [m.tree.to_c() for m in self.ligands]
+            trees = (ffi.PxdFragTree * len(self._t))(*self._t)
+        self._arrs = (mols, trees)
+        h = C.c_void_p()
+        check(load().pxd_ligand_batch_create(C.cast(mols, C.c_void_p),
+                                             C.cast(trees, C.c_void_p) if trees is not None else None,
+                                             len(self._c), C.byref(h)))
+        self.h = h
+        self.natoms = np.array([m.n_atoms for m in self.ligands], np.int32)
+        self.types = sorted(set(int(t) for m in self.ligands for t in m.vdw_type))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.pxd_ligand_batch_destroy(self.h)
+            self.h = None
+
+
+class PoseSet:
+    """CSR pose layout: pose p is ligand pose_lig[p] with coordinates
+    xyz[pose_atom_off[p] : pose_atom_off[p] + natoms] (FP32, on the device)."""
+
+    def __init__(self, batch, pose_lig, xyz_list, n_leaves, tors=None, device="cuda"):
+        import torch
+        self.batch = batch
+        self.pose_lig = np.ascontiguousarray(pose_lig, dtype=np.int32)
+        n = batch.natoms[self.pose_lig]
+        self.pose_atom_off = np.zeros(len(n), np.int32)
+        if len(n) > 1:
+            self.pose_atom_off[1:] = np.cumsum(n)[:-1]
+        self.n_atoms_total = int(n.sum())
+        flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in xyz_list]) \
+            if len(xyz_list) else np.zeros((0, 3))
+        assert len(flat) == self.n_atoms_total, "pose coordinates do not match the ligands"
+        self.n_poses = len(self.pose_lig)
+        self.n_leaves = n_leaves
+        self.d_lig = torch.from_numpy(self.pose_lig).to(device)
+        self.d_off = torch.from_numpy(self.pose_atom_off).to(device)
+        self.xyz = torch.from_numpy(flat.astype(np.float32).ravel()).to(device)
+        t = np.zeros((self.n_poses, max(n_leaves, 1)), np.float32) if tors is None else \
+            np.asarray(tors, np.float32).reshape(self.n_poses, -1)
+        self.tors = torch.from_numpy(np.ascontiguousarray(t[:, :max(n_leaves, 1)])).to(device)
+
+    def xyz_of(self, p, arr=None):
+        a = (self.xyz if arr is None else arr).detach().cpu().numpy().reshape(-1, 3)
+        o = self.pose_atom_off[p]
+        return a[o:o + self.batch.natoms[self.pose_lig[p]]]
+
+
+def score_grad(receptor, grid, poses):
+    """pxd_score_grad_batch: energies (FP64), ligand gradients, torsion gradients."""
+    import torch
+    dev = poses.xyz.device
+    e = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    g = torch.empty_like(poses.xyz)
+    tg = torch.empty((poses.n_poses, max(poses.n_leaves, 1)), dtype=torch.float32, device=dev)
+    check(load().pxd_score_grad_batch(receptor.h, grid.h if grid is not None else None,
+                                      poses.batch.h, poses.n_poses, _dev(poses.d_lig),
+                                      _dev(poses.d_off), _dev(poses.xyz), _dev(poses.tors),
+                                      _dev(e), _dev(g), _dev(tg), _stream()))
+    return e, g, tg
+
+
+def lbfgs(receptor, grid, poses, params):
+    """pxd_lbfgs_batch: optimises poses.xyz / poses.tors in place."""
+    import torch
+    dev = poses.xyz.device
+    e = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    ne = torch.empty(poses.n_poses, dtype=torch.int32, device=dev)
+    cv = torch.empty(poses.n_poses, dtype=torch.int32, device=dev)
+    check(load().pxd_lbfgs_batch(receptor.h, grid.h if grid is not None else None, poses.batch.h,
+                                 _ref(params), poses.n_poses, _dev(poses.d_lig), _dev(poses.d_off),
+                                 _dev(poses.xyz), _dev(poses.tors), _dev(e), _dev(ne), _dev(cv),
+                                 _stream()))
+    return e, ne, cv
+
+
+def lbfgs_plain(receptor, grid, poses, params):
+    """pxd_lbfgs_plain: host-driven L-BFGS, one pose per score-kernel launch."""
+    import torch
+    dev = poses.xyz.device
+    e = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    ne = torch.empty(poses.n_poses, dtype=torch.int32, device=dev)
+    cv = torch.empty(poses.n_poses, dtype=torch.int32, device=dev)
+    check(load().pxd_lbfgs_plain(receptor.h, grid.h if grid is not None else None, poses.batch.h,
+                                 _ref(params), poses.n_poses, _dev(poses.d_lig), _dev(poses.d_off),
+                                 _dev(poses.xyz), _dev(poses.tors), _dev(e), _dev(ne), _dev(cv),
+                                 _stream()))
+    return e, ne, cv
+
+
+def pscore(receptor, poses, params, with_terms=True):
+    """pxd_pscore_batch: weighted pose score and the unweighted leaf terms."""
+    import torch
+    dev = poses.xyz.device
+    s = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    t = torch.empty((poses.n_poses, ffi.NUM_TERMS), dtype=torch.float64, device=dev) \
+        if with_terms else None
+    check(load().pxd_pscore_batch(receptor.h, poses.batch.h, _ref(params), poses.n_poses,
+                                  _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
+                                  _dev(poses.tors), _dev(s), _dev(t), _stream()))
+    return s, t

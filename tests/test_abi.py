@@ -1,0 +1,41 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry
+point that include/*.h declares (no compute calls here)."""
+import ctypes as C
+import glob
+import os
+import re
+
+from conftest import ROOT
+from pxdock_amd import lib
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(pxd_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib.load()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    missing = [s for s in sorted(syms) if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_error_channel_and_host_helpers():
+    L = lib.load()
+    assert L.pxd_version().decode().startswith("pxdock")
+    n = C.c_int32(-1)
+    assert L.pxd_device_count(C.byref(n)) == 0 and n.value >= 0
+    # argument validation happens before any device work
+    assert L.pxd_receptor_create(None, None, None, None) == 1
+    assert b"null" in L.pxd_last_error()
+    b = lib.decide_grid_dims([0, 0, 0], [22.5] * 3, 0.375)
+    assert b.npts == (61, 61, 61)
+    b = lib.decide_grid_dims([1, 2, 3], [40.0] * 3, 0.375)
+    assert b.npts == (109, 109, 109)  # 107 voxels rounded up to even (grid.cpp:78)
+    assert abs(b.init_xyz[0] - (1 - 108 * 0.375 / 2)) < 1e-12

@@ -1,0 +1,149 @@
+"""The oracle against the reference's own compiled sources (oracle/_ref).
+
+oracle/_ref/libpxd_ref.so is built by oracle/Makefile from the reference's
+scorer.cpp, math.cpp, random.cpp and pair.h (the parts of the path that
+compile with the standard library alone).  These tests pin the restated
+pscore leaves, pair terms and geometry on realistic synthetic inputs that no
+gtest vector covers.  Skipped where the reference tree was not available to
+build _ref.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pxdock_amd import ffi, scoring, synth
+
+pytestmark = pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built")
+
+
+def test_vdw_table_equals_reference():
+    s, e = np.zeros(99), np.zeros(99)
+    O.ref().ref_vdw_table(O.dp(s), O.dp(e))
+    np.testing.assert_array_equal(s, np.ctypeslib.as_array(O.lib().or_gaff2_sigma if False else
+                                                           _table("or_gaff2_sigma")))
+    np.testing.assert_array_equal(e, _table("or_gaff2_epsilon"))
+
+
+def _table(name):
+    import ctypes as C
+    arr = (C.c_double * 99).in_dll(O.lib(), name)
+    return np.array(arr[:])
+
+
+def test_pair_terms_equal_reference():
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        r = rng.normal(size=3) * rng.uniform(0.5, 6)
+        d = float(np.linalg.norm(r))
+        q1, q2 = rng.normal(size=2)
+        s1, s2 = rng.uniform(0.5, 4, size=2)
+        e1, e2 = rng.uniform(0.0, 0.3, size=2)
+        sc = rng.choice([1.0, 0.5, 0.833333333])
+        ga, gb, ra, rb = (np.zeros(3) for _ in range(4))
+        a = O.lib().or_coul_pair(q1, q2, O.dp(O.f64(r)), d, sc, O.dp(ga), O.dp(gb))
+        b = O.ref().ref_coul_pair(q1, q2, O.dp(O.f64(r)), d, sc, O.dp(ra), O.dp(rb))
+        assert a == pytest.approx(b, rel=1e-14, abs=1e-300)
+        np.testing.assert_allclose(ga, ra, rtol=1e-13, atol=1e-300)
+        ga[:] = gb[:] = ra[:] = rb[:] = 0
+        a = O.lib().or_vdw_pair(s1, e1, s2, e2, O.dp(O.f64(r)), d, sc, O.dp(ga), O.dp(gb))
+        b = O.ref().ref_vdw_pair(s1, e1, s2, e2, O.dp(O.f64(r)), d, sc, O.dp(ra), O.dp(rb))
+        assert a == pytest.approx(b, rel=1e-13, abs=1e-300)
+        np.testing.assert_allclose(gb, rb, rtol=1e-12, atol=1e-300)
+
+
+def test_geometry_equals_reference():
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        p = [O.f64(rng.normal(size=3) * 2) for _ in range(4)]
+        g = [np.zeros(3) for _ in range(4)]
+        h = [np.zeros(3) for _ in range(4)]
+        a = O.lib().or_dihedral_grad(*[O.dp(x) for x in p], *[O.dp(x) for x in g])
+        b = O.ref().ref_dihedral_grad(*[O.dp(x) for x in p], *[O.dp(x) for x in h])
+        assert a == pytest.approx(b, abs=1e-13)
+        for x, y in zip(g, h):
+            np.testing.assert_allclose(x, y, rtol=1e-10, atol=1e-12)
+        a = O.lib().or_angle_grad(O.dp(p[0]), O.dp(p[1]), O.dp(p[2]), O.dp(g[0]), O.dp(g[1]))
+        b = O.ref().ref_angle_grad(O.dp(p[0]), O.dp(p[1]), O.dp(p[2]), O.dp(h[0]), O.dp(h[1]))
+        assert a == pytest.approx(b, abs=1e-13)
+        ax = p[3] / np.linalg.norm(p[3])
+        m1, j1, m2, j2 = (np.zeros(9) for _ in range(4))
+        t = float(rng.uniform(-4, 4))
+        O.lib().or_rotmat_axis_grad(O.dp(ax), t, O.dp(m1), O.dp(j1))
+        O.ref().ref_rotmat_axis_grad(O.dp(ax), t, O.dp(m2), O.dp(j2))
+        np.testing.assert_allclose(m1, m2, atol=1e-15)
+        np.testing.assert_allclose(j1, j2, atol=1e-15)
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        O.lib().or_rotmat_quat(O.dp(O.f64(q)), O.dp(m1))
+        O.ref().ref_rotmat_quat(O.dp(O.f64(q)), O.dp(m2))
+        np.testing.assert_allclose(m1, m2, atol=1e-15)
+
+
+@pytest.fixture(scope="module")
+def receptor():
+    return synth.make_receptor(1)
+
+
+def _near_pocket_poses(rec, lig, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        c = rec.pocket_center + rng.normal(size=3) * 2.0
+        R = synth.quat_to_rot(synth.random_quaternion(rng))
+        t = rng.uniform(-np.pi, np.pi, size=lig.tree.n_torsions)
+        out.append(synth.ligand_apply(lig.tree, lig.xyz[0], c, R, t))
+    return out
+
+
+@pytest.mark.parametrize("seed", [1000, 1001, 1002, 1003])
+def test_pscore_equals_reference(receptor, seed):
+    """Every pose_selection leaf of the default scoring JSON, oracle vs reference."""
+    sp = scoring.load_pose_selection()
+    lig = synth.make_ligand(seed, n_heavy=30, n_torsions=6, n_rings=2)
+    tors = np.random.default_rng(seed).uniform(-np.pi, np.pi, receptor.leaves.n_leaves)
+    rx = O.receptor_apply(receptor.leaves, receptor.xyz, tors)
+    nonzero = np.zeros(ffi.NUM_TERMS, bool)
+    for x in _near_pocket_poses(receptor, lig, 12, seed):
+        s1, t1 = O.pscore(sp, receptor, rx, lig, x)
+        s2, t2 = O.pscore(sp, receptor, rx, lig, x, use_ref=True)
+        np.testing.assert_allclose(t1, t2, rtol=1e-10, atol=1e-10)
+        assert s1 == pytest.approx(s2, rel=1e-10, abs=1e-10)
+        nonzero |= np.abs(t1) > 0
+    # the synthetic systems exercise the leaves (rings / ions may be absent per seed)
+    assert nonzero[[0, 1, 6, 7]].all()
+
+
+def test_pscore_each_term_alone_equals_reference(receptor):
+    """One active coefficient at a time, with non-default sub-coefficients."""
+    base = scoring.load_pose_selection()
+    lig = synth.make_ligand(2024, n_heavy=40, n_torsions=8, n_rings=3)
+    names = ["coul_coef", "vdw_coef", "ionic_coef", "pication_coef", "f2f_coef", "e2f_coef",
+             "hbond_coef", "hydrophobic_coef", "tstrain_coef"]
+    for k, name in enumerate(names):
+        sp = scoring.load_pose_selection()
+        for other in names:
+            setattr(sp, other, 0.0)
+        setattr(sp, name, 0.7)
+        for x in _near_pocket_poses(receptor, lig, 4, 77 + k):
+            s1, t1 = O.pscore(sp, receptor, receptor.xyz, lig, x)
+            s2, t2 = O.pscore(sp, receptor, receptor.xyz, lig, x, use_ref=True)
+            assert s1 == pytest.approx(s2, rel=1e-10, abs=1e-12), name
+            np.testing.assert_allclose(t1, t2, rtol=1e-10, atol=1e-12)
+    del base
+
+
+def test_random_samplers_match_reference_properties():
+    """random_in_sphere / random_orientation (random.cpp:29-49): the oracle-side
+    restatement (synth.random_quaternion) and the reference draw from the same
+    distributions: |q| = 1 and mean |v| of a uniform unit ball is 3/4."""
+    n = 4000
+    sph, qs = np.zeros(3 * n), np.zeros(4 * n)
+    O.ref().ref_random_draws(59, n, O.dp(sph), O.dp(qs))
+    sph = sph.reshape(n, 3)
+    qs = qs.reshape(n, 4)
+    np.testing.assert_allclose(np.linalg.norm(qs, axis=1), 1.0, atol=1e-12)
+    assert np.linalg.norm(sph, axis=1).max() < 1.0
+    assert abs(np.linalg.norm(sph, axis=1).mean() - 0.75) < 0.02
+    ours = synth.random_quaternion(np.random.default_rng(1), n)
+    # both are uniform on SO(3): the first component's |a| has mean 2/pi * ... compare moments
+    assert abs(np.abs(ours[:, 0]).mean() - np.abs(qs[:, 0]).mean()) < 0.03
