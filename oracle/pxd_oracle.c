@@ -569,6 +569,23 @@ double or_system_energy(const or_system* s, const double* tors, const double* lx
 /* ======================================================================== */
 /* L-BFGS with strong-Wolfe line search (optimizer.cpp:56-560)               */
 
+static __thread double g_noise_rel = 0.0;
+static __thread uint64_t g_noise_state = 0;
+
+void or_set_eval_noise(double rel, uint64_t seed) {
+  g_noise_rel = rel;
+  g_noise_state = seed * 0x9E3779B97F4A7C15ull + 1;
+}
+
+/* splitmix64 -> uniform in [-1, 1) */
+static double noise_u(void) {
+  uint64_t z = (g_noise_state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) / 4503599627370496.0 - 1.0;
+}
+
 typedef struct {
   const or_system* s;
   int n, nl3, nt;
@@ -581,6 +598,10 @@ typedef struct {
 static double eval_dir(closure* c, double t, const double* d, double* g) {
   for (int i = 0; i < c->n; ++i) c->xt[i] = c->x0[i] + d[i] * t;
   double e = or_system_energy(c->s, c->xt + c->nl3, c->xt, g + c->nl3, g);
+  if (g_noise_rel != 0.0) {
+    e *= 1.0 + g_noise_rel * noise_u();
+    for (int i = 0; i < c->n; ++i) g[i] *= 1.0 + g_noise_rel * noise_u();
+  }
   ++*c->nevals;
   return e;
 }

@@ -94,6 +94,10 @@ double or_system_energy(const or_system* s, const double* tors, const double* li
                         double* tors_grad, double* lig_grad);
 
 /* ---- L-BFGS (optimizer.cpp:205-560) ---- */
+/* Test-only: multiply every energy / gradient component the L-BFGS closure
+ * sees by (1 + rel * u), u uniform in [-1, 1) from a seeded stream, to model
+ * FP32 evaluation error.  rel = 0 (the default) disables it.  Per thread. */
+void or_set_eval_noise(double rel, uint64_t seed);
 /* Optimises (lig_xyz, tors) in place; returns converged; energy/nevals out. */
 int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lig_xyz,
                  double* tors, double* energy, int32_t* nevals);

@@ -64,6 +64,7 @@ def lib():
             "or_philox4x32": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                      C.POINTER(C.c_uint32)]),
             "or_pose_cluster": (i, [i, i, f64p, f64p, i32p, i, d, i32p]),
+            "or_set_eval_noise": (None, [d, C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -480,10 +480,12 @@ static pxd_status grid_alloc(const pxd_receptor* r, const pxd_box* box, const in
   DevGrid& D = g->dev;
   for (int d = 0; d < 3; ++d) {
     D.init[d] = (float)box->init_xyz[d];
+    D.init_d[d] = box->init_xyz[d];
     D.np[d] = box->npoints[d];
   }
   D.spacing = (float)box->spacing;
   D.inv_spacing = (float)(1.0 / box->spacing);
+  D.inv_spacing_d = 1.0 / box->spacing;
   D.slope = (float)slope;
   D.P = P;
   D.maps = reinterpret_cast<float4*>(static_cast<char*>(g->mem) + kGridHeader);

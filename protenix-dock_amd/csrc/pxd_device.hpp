@@ -475,27 +475,30 @@ __device__ inline float4 interp(const float4* map, const long long* off, const f
 // receptor_cache::get (grid.cpp:197-262) for one ligand atom: adds into e/g.
 __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float* x,
                                 double* e, float* g) {
+  // The cell coordinate is formed in FP64: in FP32 its fractional part would
+  // carry ~1e-6 absolute error, i.e. a large relative error on the small
+  // trilinear weights that multiply steep (clash) corner values.
   float f[3], gt[3], pen = 0.f;
   int lo[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    f[d] = (x[d] - G.init[d]) * G.inv_spacing;
+    double fd = ((double)x[d] - G.init_d[d]) * G.inv_spacing_d;
     int nm1 = G.np[d] - 1;
-    if (f[d] < 0.f) {
+    if (fd < 0.0) {
       lo[d] = 0;
       gt[d] = -1.f;
-      pen -= f[d];
-      f[d] = 0.f;
-    } else if (f[d] < (float)nm1) {
-      lo[d] = (int)f[d];
+      pen -= (float)fd;
+      fd = 0.0;
+    } else if (fd < (double)nm1) {
+      lo[d] = (int)fd;
       gt[d] = 0.f;
     } else {
       lo[d] = nm1 - 1;
       gt[d] = 1.f;
-      pen += f[d] - (float)nm1;
-      f[d] = (float)nm1;
+      pen += (float)(fd - (double)nm1);
+      fd = (double)nm1;
     }
-    f[d] -= (float)lo[d];
+    f[d] = (float)(fd - (double)lo[d]);
   }
   long long ny = G.np[1], nz = G.np[2];
   long long base = ((long long)lo[0] * ny + lo[1]) * nz + lo[2];

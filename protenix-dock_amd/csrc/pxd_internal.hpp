@@ -129,6 +129,8 @@ struct DevGrid {
   float init[3];
   int np[3];
   float spacing, inv_spacing;
+  double init_d[3];       // FP64 copies for the cell coordinate of a lookup
+  double inv_spacing_d;
   float slope;
   long long P;                // points per map
   const float4* maps;         // [(n_types+1)][P], map 0 = Coulomb

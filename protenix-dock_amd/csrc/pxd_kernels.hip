@@ -385,6 +385,158 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
   vcopy<VPL>(c.vec(V_G), c.vec(idx.get(low)));
 }
 
+// lbfgs_step::apply (optimizer.cpp:429-560) on the pose held in the context:
+// x = c.vec(V_X) is the start (ligand xyz then receptor torsions, c.n3 / c.n
+// set, c.nevals = 0); on return x is the result.  hy / hs: this wave's history
+// scratch (history x 64*VPL floats each).
+template <int VPL>
+__device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy, float* hs,
+                            double* s_ro, double* s_al, double* energy_out,
+                            bool* converged_out) {
+  const int lane = lane_id();
+  const int m = P.history;
+  constexpr int NV = kWave * VPL;
+  float* x = c.vec(V_X);
+  float* g = c.vec(V_G);
+  float* gp = c.vec(V_GP);
+  float* d = c.vec(V_D);
+  float* y = c.vec(V_W0);
+  float* sv = c.vec(V_W1);
+  float* q = c.vec(V_W2);
+#pragma unroll
+  for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
+  // initial f(x), df/dx (optimizer.cpp:435-441)
+  double loss = eval_dir(c, 0.0, d, g);
+  double energy = loss;  // kept unless a step is taken (optimizer.cpp:554-558)
+  bool converged = false, updated = false;
+  if (vamax<VPL>(g) < (float)P.tol_grad) {
+    converged = true;
+  } else {
+    int head = 0, num_old = 0, n_iter = 0;
+    bool high_change = false;
+    double H = 1.0, t = 0.0, prev_loss;
+    while (n_iter < P.max_iter && c.nevals < P.max_eval) {
+      n_iter++;
+      if (n_iter > 1) {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) y[lane + kWave * k] = g[lane + kWave * k] - gp[lane + kWave * k];
+        high_change = vamax<VPL>(y) > 1e8f;
+      }
+      if (n_iter == 1 || high_change) {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = -g[lane + kWave * k];
+        num_old = 0;
+        head = 0;
+        H = 1.0;
+      } else {
+        float tf = (float)t;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) sv[lane + kWave * k] = d[lane + kWave * k] * tf;
+        double ys = vdot<VPL>(y, sv);
+        if (ys > 1e-10) {  // optimizer.cpp:473-487
+          int slot;
+          if (num_old == m) {
+            slot = head;
+            head = (head + 1) % m;
+          } else {
+            slot = (head + num_old) % m;
+            num_old++;
+          }
+          H = ys / vdot<VPL>(y, y);
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) {
+            hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
+            hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
+          }
+          if (lane == 0) s_ro[slot] = 1.0 / ys;
+          __syncthreads();
+        }
+        // two-loop recursion (optimizer.cpp:493-507)
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) q[lane + kWave * k] = -g[lane + kWave * k];
+        for (int h = num_old - 1; h >= 0; --h) {
+          int slot = (head + h) % m;
+          const float* sh = hs + (size_t)slot * NV;
+          const float* yh = hy + (size_t)slot * NV;
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k)
+            acc += (double)sh[lane + kWave * k] * (double)q[lane + kWave * k];
+          double al = wave_sum(acc) * s_ro[slot];
+          if (lane == 0) s_al[h] = al;
+          float alf = (float)al;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) q[lane + kWave * k] -= alf * yh[lane + kWave * k];
+        }
+        __syncthreads();
+        float Hf = (float)H;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = q[lane + kWave * k] * Hf;
+        for (int h = 0; h < num_old; ++h) {
+          int slot = (head + h) % m;
+          const float* sh = hs + (size_t)slot * NV;
+          const float* yh = hy + (size_t)slot * NV;
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k)
+            acc += (double)yh[lane + kWave * k] * (double)d[lane + kWave * k];
+          double be = wave_sum(acc) * s_ro[slot];
+          float cf = (float)(s_al[h] - be);
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] += sh[lane + kWave * k] * cf;
+        }
+      }
+      vcopy<VPL>(gp, g);
+      prev_loss = loss;
+      if (n_iter == 1 || high_change) {
+        double l1 = 0.0;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) l1 += fabs((double)g[lane + kWave * k]);
+        l1 = wave_sum(l1);
+        double inv = 1.0 / l1;
+        t = (1.0 < inv ? 1.0 : inv) * P.lr * 0.1;
+      } else {
+        t = P.lr;
+      }
+      double gtd = vdot<VPL>(g, d);
+      if (gtd > -P.tol_change) {
+        converged = true;
+        break;
+      }
+      strong_wolfe<VPL>(c, P, &t, d, &loss, gtd);
+      if (isnan(loss)) break;
+      float tf = (float)t;
+      float smax = 0.f;
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) {
+        x[lane + kWave * k] += d[lane + kWave * k] * tf;
+        smax = fmaxf(smax, fabsf(d[lane + kWave * k] * tf));
+      }
+      updated = true;
+      if (vamax<VPL>(g) <= (float)P.tol_grad) {
+        converged = true;
+        break;
+      }
+      if (wave_max(smax) <= (float)P.tol_change) {
+        converged = true;
+        break;
+      }
+      if (fabs(loss - prev_loss) < P.tol_change) {
+        converged = true;
+        break;
+      }
+    }
+    // optimizer.cpp:553-558: one final evaluation only after a step
+    if (updated) {
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
+      energy = eval_dir(c, 0.0, d, g);
+    }
+  }
+  *energy_out = energy;
+  *converged_out = converged;
+}
+
 template <int VPL>
 __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
                                               const DevGrid* __restrict__ G, LbfgsArgs A) {
@@ -394,11 +546,9 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   __shared__ double s_al[kMaxHistory];
   __shared__ int s_pose;
   const int lane = lane_id();
-  const LbfgsParamsDev& P = A.prm;
-  const int m = P.history;
   constexpr int NV = kWave * VPL;
-  float* hy = A.hist + (size_t)blockIdx.x * 2 * m * NV;
-  float* hs = hy + (size_t)m * NV;
+  float* hy = A.hist + (size_t)blockIdx.x * 2 * A.prm.history * NV;
+  float* hs = hy + (size_t)A.prm.history * NV;
   LbfgsCtx<VPL> c;
   c.R = &R;
   c.L = &L;
@@ -406,12 +556,6 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   c.s = carve_pose_lds(lds);
   c.v = vpool;
   float* x = c.vec(V_X);
-  float* g = c.vec(V_G);
-  float* gp = c.vec(V_GP);
-  float* d = c.vec(V_D);
-  float* y = c.vec(V_W0);
-  float* sv = c.vec(V_W1);
-  float* q = c.vec(V_W2);
   for (;;) {
     if (lane == 0) s_pose = atomicAdd(A.counter, 1);
     __syncthreads();
@@ -429,136 +573,10 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
       int e = lane + kWave * k;
       x[e] = e < c.n3 ? A.xyz[off + e]
                       : (e < c.n ? A.tors[(long long)pi * R.n_leaves + (e - c.n3)] : 0.f);
-      d[e] = 0.f;
     }
-    // initial f(x), df/dx (optimizer.cpp:435-441)
-    double loss = eval_dir(c, 0.0, d, g);
-    double energy = loss;  // kept unless a step is taken (optimizer.cpp:554-558)
-    bool converged = false, updated = false;
-    if (vamax<VPL>(g) < (float)P.tol_grad) {
-      converged = true;
-    } else {
-      int head = 0, num_old = 0, n_iter = 0;
-      bool high_change = false;
-      double H = 1.0, t = 0.0, prev_loss;
-      while (n_iter < P.max_iter && c.nevals < P.max_eval) {
-        n_iter++;
-        if (n_iter > 1) {
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) y[lane + kWave * k] = g[lane + kWave * k] - gp[lane + kWave * k];
-          high_change = vamax<VPL>(y) > 1e8f;
-        }
-        if (n_iter == 1 || high_change) {
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = -g[lane + kWave * k];
-          num_old = 0;
-          head = 0;
-          H = 1.0;
-        } else {
-          float tf = (float)t;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) sv[lane + kWave * k] = d[lane + kWave * k] * tf;
-          double ys = vdot<VPL>(y, sv);
-          if (ys > 1e-10) {  // optimizer.cpp:473-487
-            int slot;
-            if (num_old == m) {
-              slot = head;
-              head = (head + 1) % m;
-            } else {
-              slot = (head + num_old) % m;
-              num_old++;
-            }
-            H = ys / vdot<VPL>(y, y);
-#pragma unroll
-            for (int k = 0; k < VPL; ++k) {
-              hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
-              hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
-            }
-            if (lane == 0) s_ro[slot] = 1.0 / ys;
-            __syncthreads();
-          }
-          // two-loop recursion (optimizer.cpp:493-507)
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) q[lane + kWave * k] = -g[lane + kWave * k];
-          for (int h = num_old - 1; h >= 0; --h) {
-            int slot = (head + h) % m;
-            const float* sh = hs + (size_t)slot * NV;
-            const float* yh = hy + (size_t)slot * NV;
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              acc += (double)sh[lane + kWave * k] * (double)q[lane + kWave * k];
-            double al = wave_sum(acc) * s_ro[slot];
-            if (lane == 0) s_al[h] = al;
-            float alf = (float)al;
-#pragma unroll
-            for (int k = 0; k < VPL; ++k) q[lane + kWave * k] -= alf * yh[lane + kWave * k];
-          }
-          __syncthreads();
-          float Hf = (float)H;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = q[lane + kWave * k] * Hf;
-          for (int h = 0; h < num_old; ++h) {
-            int slot = (head + h) % m;
-            const float* sh = hs + (size_t)slot * NV;
-            const float* yh = hy + (size_t)slot * NV;
-            double acc = 0.0;
-#pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              acc += (double)yh[lane + kWave * k] * (double)d[lane + kWave * k];
-            double be = wave_sum(acc) * s_ro[slot];
-            float cf = (float)(s_al[h] - be);
-#pragma unroll
-            for (int k = 0; k < VPL; ++k) d[lane + kWave * k] += sh[lane + kWave * k] * cf;
-          }
-        }
-        vcopy<VPL>(gp, g);
-        prev_loss = loss;
-        if (n_iter == 1 || high_change) {
-          double l1 = 0.0;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) l1 += fabs((double)g[lane + kWave * k]);
-          l1 = wave_sum(l1);
-          double inv = 1.0 / l1;
-          t = (1.0 < inv ? 1.0 : inv) * P.lr * 0.1;
-        } else {
-          t = P.lr;
-        }
-        double gtd = vdot<VPL>(g, d);
-        if (gtd > -P.tol_change) {
-          converged = true;
-          break;
-        }
-        strong_wolfe<VPL>(c, P, &t, d, &loss, gtd);
-        if (isnan(loss)) break;
-        float tf = (float)t;
-        float smax = 0.f;
-#pragma unroll
-        for (int k = 0; k < VPL; ++k) {
-          x[lane + kWave * k] += d[lane + kWave * k] * tf;
-          smax = fmaxf(smax, fabsf(d[lane + kWave * k] * tf));
-        }
-        updated = true;
-        if (vamax<VPL>(g) <= (float)P.tol_grad) {
-          converged = true;
-          break;
-        }
-        if (wave_max(smax) <= (float)P.tol_change) {
-          converged = true;
-          break;
-        }
-        if (fabs(loss - prev_loss) < P.tol_change) {
-          converged = true;
-          break;
-        }
-      }
-      // optimizer.cpp:553-558: one final evaluation only after a step
-      if (updated) {
-#pragma unroll
-        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
-        energy = eval_dir(c, 0.0, d, g);
-      }
-    }
+    double energy;
+    bool converged;
+    lbfgs_solve<VPL>(c, A.prm, hy, hs, s_ro, s_al, &energy, &converged);
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
       int e = lane + kWave * k;
