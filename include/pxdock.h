@@ -307,6 +307,56 @@ pxd_status pxd_lbfgs_plain(const pxd_receptor* r, const pxd_grid* grid,
                            double* energy, int32_t* nevals, int32_t* converged,
                            void* stream);
 
+/* Monte-Carlo search (monte_carlo_searcher, searcher.h:27-82). */
+typedef struct pxd_mc_params {
+  uint64_t seed;            /* walker k of ligand l draws Philox4x32-10 blocks with key
+                               (seed_lo, seed_hi + k) and counter (draw, l, 0, 0) */
+  int32_t max_nsteps;       /* Monte-Carlo steps per walker (bdock default 200) */
+  int32_t mutation;         /* 0: reference mutate_inplace (searcher.cpp:189-223);
+                               1: Gaussian translation + rotation and one torsion per step */
+  double amplitude;         /* 2 A (searcher.h:33) */
+  double temperature;       /* 1.2 (searcher.h:33) */
+  double trans_sigma;       /* mutation 1 only */
+  double rot_sigma;         /* mutation 1 only */
+  int32_t max_tries;        /* randomize attempts (searcher.cpp:73-77): 2000 */
+  int32_t steps_per_launch; /* steps per kernel launch of the captured step loop */
+  pxd_lbfgs_params relax;   /* after each mutation: relax_nsteps (40) iterations */
+  pxd_lbfgs_params full;    /* initial and refine: 240 iterations */
+} pxd_mc_params;
+
+/* Reference defaults (bdock.cpp:60-78, searcher.h:33). */
+void pxd_mc_default_params(pxd_mc_params* p);
+
+/* Docking search for every ligand of the batch with `walkers` walkers each
+ * (exhaustiveness); walker 0 of a ligand optimises the input pose and does no
+ * Monte-Carlo (searcher.cpp:105-123).  All buffers are DEVICE memory:
+ *   lig_xyz0   [sum_l natoms_l][3] input conformation of each ligand (batch order)
+ *   box        search box (check_atom_in_box, grid.h:42-46): the grid box when
+ *              maps are used, describe_boundary otherwise
+ *   best_xyz   [sum_l walkers*natoms_l][3] best pose per walker (ligand-major)
+ *   best_tors  [n_walkers][n_leaves], best_energy / best_pscore [n_walkers] (FP64),
+ *   nevals     [n_walkers] (0 marks a failed walker, searcher.cpp:178-181)
+ *   cluster_idx [n_ligands][num_modes] walker index within the ligand (-1 pad),
+ *   cluster_count [n_ligands]: pose_cluster::preprocess (writer.cpp:124-177).
+ * The step loop is captured once in a hipGraph and replayed. */
+pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* grid,
+                         const pxd_ligand_batch* b, const pxd_score_params* sp,
+                         const pxd_mc_params* prm, const pxd_box* box, int32_t walkers,
+                         const float* lig_xyz0, float* best_xyz, float* best_tors,
+                         double* best_energy, double* best_pscore, int32_t* nevals,
+                         int32_t num_modes, double min_rmsd, int32_t* cluster_idx,
+                         int32_t* cluster_count, void* stream);
+
+/* free_ligand::apply_parameters (data.cpp:592-641) for a batch of poses:
+ * pose p of ligand pose_lig[p] is rebuilt from xyz_in (its input conformation)
+ * with centre center[p][3], orientation matrix rot[p][9] (row major) and the
+ * torsion rotations tors[tors_off[p] .. tors_off[p+1]) in the tree's
+ * pre-order.  DEVICE buffers; xyz_out uses the xyz_in layout. */
+pxd_status pxd_apply_poses(const pxd_ligand_batch* b, int32_t n_poses, const int32_t* pose_lig,
+                           const int32_t* pose_atom_off, const float* xyz_in,
+                           const float* center, const float* rot, const int32_t* tors_off,
+                           const float* tors, float* xyz_out, void* stream);
+
 /* Batched pscore (root_scorer::combine on the pose_selection tree,
  * scorer.cpp:117-205 and leaves :207-613) for n_poses poses.  Receptor
  * coordinates come from applying tors to the receptor's leaves

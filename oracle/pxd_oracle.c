@@ -1438,3 +1438,231 @@ int32_t or_pose_cluster(int32_t n_cand, int32_t n, const double* xyz, const doub
   }
   return cnt;
 }
+
+/* ======================================================================== */
+/* Monte-Carlo walker (searcher.cpp:46-223) with the Philox draws of the GPU */
+/* search: key (seed_lo, seed_hi + walker), counter (draw, ligand, 0, 0).    */
+
+typedef struct mc_rng {
+  uint32_t key[2];
+  uint32_t ligand, ctr;
+} mc_rng;
+
+static void rng_block(mc_rng* r, uint32_t w[4]) {
+  uint32_t c[4] = {r->ctr++, r->ligand, 0u, 0u};
+  or_philox4x32(c, r->key, w);
+}
+
+static double rng_u01(mc_rng* r) {
+  uint32_t w[4];
+  rng_block(r, w);
+  return (double)w[0] * 2.3283064365386963e-10; /* 2^-32 */
+}
+
+static double rng_uniform(mc_rng* r, double a, double b) { return a + (b - a) * rng_u01(r); }
+
+static int rng_uniform_int(mc_rng* r, int a, int b) {
+  int v = a + (int)floor(rng_u01(r) * (double)(b - a + 1));
+  return v > b ? b : v;
+}
+
+static void rng_in_sphere(mc_rng* r, double v[3]) { /* random.cpp:42-49 */
+  for (;;) {
+    uint32_t w[4];
+    rng_block(r, w);
+    for (int d = 0; d < 3; ++d) v[d] = (double)w[d] * 4.656612873077393e-10 - 1.0;
+    if (sqsum3(v) < 1.0) return;
+  }
+}
+
+static void rng_quaternion(mc_rng* r, double q[4]) { /* random.cpp:29-40 */
+  uint32_t w[4];
+  rng_block(r, w);
+  double u1 = (double)w[0] * 2.3283064365386963e-10;
+  double u2 = (double)w[1] * 2.3283064365386963e-10 * 6.283185307179586;
+  double u3 = (double)w[2] * 2.3283064365386963e-10 * 6.283185307179586;
+  double s1 = sqrt(1.0 - u1), s2 = sqrt(u1);
+  q[0] = s1 * sin(u2);
+  q[1] = s1 * cos(u2);
+  q[2] = s2 * sin(u3);
+  q[3] = s2 * cos(u3);
+}
+
+static double rng_normal(mc_rng* r) { /* Box-Muller */
+  uint32_t w[4];
+  rng_block(r, w);
+  double u1 = ((double)w[0] + 1.0) * 2.3283064365386963e-10;
+  double u2 = (double)w[1] * 2.3283064365386963e-10;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+static int all_in_box(const double* x, int n, const pxd_box* b) { /* grid.h:42-46 */
+  for (int a = 0; a < n; ++a)
+    for (int d = 0; d < 3; ++d)
+      if (!(x[3 * a + d] > b->init_xyz[d] && x[3 * a + d] < b->oppo_xyz[d])) return 0;
+  return 1;
+}
+
+static void centroid3(const double* x, int n, double c[3]) {
+  c[0] = c[1] = c[2] = 0.0;
+  for (int a = 0; a < n; ++a) axpy3(c, 1.0, x + 3 * a);
+  scale3(c, 1.0 / n);
+}
+
+static int coords_sane(const double* x, int n) { /* searcher.cpp:89-96 */
+  for (int k = 0; k < 3 * n; ++k)
+    if (!(x[k] <= 1e8 && x[k] >= -1e8)) return 0;
+  return 1;
+}
+
+static double mc_pscore(const or_system* s, const pxd_score_params* sp, const double* lx,
+                        const double* tors) {
+  int nr = s->rec->n_atoms;
+  double* rx = (double*)malloc(sizeof(double) * 3 * nr);
+  if (s->leaves && s->leaves->n_leaves)
+    or_receptor_apply(s->leaves, nr, s->rec_xyz, tors, rx); /* searcher.cpp:119 */
+  else
+    memcpy(rx, s->rec_xyz, sizeof(double) * 3 * nr);
+  double v = or_pscore(sp, s->rec, rx, s->lig, lx, NULL);
+  free(rx);
+  return v;
+}
+
+/* randomize (searcher.cpp:46-87) from the input conformation xyz0. */
+static void mc_randomize(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
+                         const pxd_mc_params* prm, const pxd_box* box, mc_rng* r,
+                         const double* xyz0, double* out) {
+  int n = s->lig->n_atoms;
+  double tors[PXD_MAX_FRAGMENTS];
+  for (int tries = 1;; ++tries) {
+    double cen[3], q[4], rot[9];
+    for (int d = 0; d < 3; ++d) cen[d] = rng_uniform(r, box->init_xyz[d], box->oppo_xyz[d]);
+    rng_quaternion(r, q);
+    for (int k = 0; k < t->n_torsions; ++k)
+      tors[k] = rng_uniform(r, -3.141592653589793, 3.141592653589793);
+    or_rotmat_quat(q, rot);
+    or_ligand_apply(t, n, xyz0, cen, rot, tors, out);
+    if (tries < prm->max_tries && !all_in_box(out, n, box)) continue;
+    double cc[3];
+    centroid3(out, n, cc);
+    /* a centroid outside the box is retried; bounded at 4x max_tries */
+    if (!all_in_box(cc, 1, box) && tries < 4 * prm->max_tries) continue;
+    if (tries >= prm->max_tries) break;
+    if (or_pscore_vdw(sp, s->rec, s->rec_xyz, s->lig, out) < 100.0) break;
+  }
+}
+
+/* mutate_inplace (searcher.cpp:189-223), and the Gaussian variant
+ * (mutation 1: translation and rotation-vector N(0, sigma^2), one torsion). */
+static void mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, mc_rng* r, int n,
+                      double* x) {
+  double cen[3], rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, tors[PXD_MAX_FRAGMENTS] = {0};
+  centroid3(x, n, cen);
+  if (prm->mutation == 0) {
+    int which = rng_uniform_int(r, 0, t->n_torsions + 1);
+    if (which == 0) {
+      double dl[3];
+      rng_in_sphere(r, dl);
+      axpy3(cen, prm->amplitude, dl);
+    } else if (which == 1) {
+      double gr = 0.0;
+      for (int a = 0; a < n; ++a) {
+        double v[3];
+        sub3(x + 3 * a, cen, v);
+        gr += sqsum3(v);
+      }
+      gr = sqrt(gr / n);
+      if (gr < 1e-6) return; /* searcher.cpp:207 */
+      double dl[3];
+      rng_in_sphere(r, dl);
+      scale3(dl, prm->amplitude / gr);
+      double ang = norm3(dl);
+      scale3(dl, 1.0 / ang);
+      or_rotmat_axis(dl, ang, rot);
+    } else {
+      tors[which - 2] = rng_uniform(r, -3.141592653589793, 3.141592653589793);
+    }
+  } else {
+    for (int d = 0; d < 3; ++d) cen[d] += rng_normal(r) * prm->trans_sigma;
+    double v[3];
+    for (int d = 0; d < 3; ++d) v[d] = rng_normal(r) * prm->rot_sigma;
+    double ang = norm3(v);
+    if (ang > 1e-12) {
+      scale3(v, 1.0 / ang);
+      or_rotmat_axis(v, ang, rot);
+    }
+    if (t->n_torsions > 0) {
+      int k = rng_uniform_int(r, 0, t->n_torsions - 1);
+      tors[k] = rng_uniform(r, -3.141592653589793, 3.141592653589793);
+    }
+  }
+  double* y = (double*)malloc(sizeof(double) * 3 * n);
+  or_ligand_apply(t, n, x, cen, rot, tors, y);
+  memcpy(x, y, sizeof(double) * 3 * n);
+  free(y);
+}
+
+/* monte_carlo_searcher::mutate_and_optimize (searcher.cpp:98-187) for one
+ * walker.  Returns the number of Monte-Carlo steps taken. */
+int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
+                     const pxd_mc_params* prm, const pxd_box* box, int32_t walker,
+                     int32_t ligand, const double* xyz0, double* best_xyz, double* best_tors,
+                     double* best_e, double* best_ps, int32_t* nevals) {
+  const int n = s->lig->n_atoms, n3 = 3 * n;
+  const int nt = s->leaves ? s->leaves->n_leaves : 0;
+  mc_rng r = {{(uint32_t)prm->seed, (uint32_t)(prm->seed >> 32) + (uint32_t)walker},
+              (uint32_t)ligand, 0u};
+  double* cur = (double*)malloc(sizeof(double) * 3 * n);
+  double* cdd = (double*)malloc(sizeof(double) * 3 * n);
+  double ct[PXD_MAX_LEAVES + 1] = {0}, dt[PXD_MAX_LEAVES + 1];
+  if (walker == 0)
+    memcpy(cur, xyz0, sizeof(double) * n3);
+  else
+    mc_randomize(s, t, sp, prm, box, &r, xyz0, cur);
+  double e, cur_ps;
+  int32_t ne = 0;
+  or_lbfgs(s, &prm->full, cur, ct, &e, &ne);
+  cur_ps = mc_pscore(s, sp, cur, ct);
+  *nevals = ne;
+  *best_e = e;
+  *best_ps = cur_ps;
+  memcpy(best_xyz, cur, sizeof(double) * n3);
+  if (nt) memcpy(best_tors, ct, sizeof(double) * nt);
+  int steps = 0;
+  if (walker != 0) {
+    if (all_in_box(cur, n, box)) {
+      for (int step = 0; step < prm->max_nsteps; ++step, ++steps) {
+        memcpy(cdd, cur, sizeof(double) * n3);
+        memcpy(dt, ct, sizeof(dt));
+        mc_mutate(t, prm, &r, n, cdd);
+        double ec, pc;
+        or_lbfgs(s, &prm->relax, cdd, dt, &ec, &ne);
+        *nevals += ne;
+        pc = mc_pscore(s, sp, cdd, dt);
+        /* Metropolis on pscore (searcher.cpp:164-165, searcher.h:53-56) */
+        int acc = step == 0 || pc < cur_ps ||
+                  rng_u01(&r) < exp((cur_ps - pc) / prm->temperature);
+        if (!(acc && all_in_box(cdd, n, box))) continue;
+        memcpy(cur, cdd, sizeof(double) * n3);
+        memcpy(ct, dt, sizeof(dt));
+        cur_ps = pc;
+        if (cur_ps < *best_ps) { /* refine (searcher.cpp:167-174) */
+          or_lbfgs(s, &prm->full, cur, ct, &ec, &ne);
+          *nevals += ne;
+          cur_ps = mc_pscore(s, sp, cur, ct);
+          if (cur_ps < *best_ps) {
+            *best_ps = cur_ps;
+            *best_e = ec;
+            memcpy(best_xyz, cur, sizeof(double) * n3);
+            if (nt) memcpy(best_tors, ct, sizeof(double) * nt);
+          }
+        }
+      }
+    } else if (!coords_sane(cur, n)) {
+      *nevals = 0; /* searcher.cpp:178-181 */
+    }
+  }
+  free(cur);
+  free(cdd);
+  return steps;
+}

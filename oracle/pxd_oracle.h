@@ -134,6 +134,16 @@ int32_t or_pose_cluster(int32_t n_cand, int32_t n_atoms, const double* xyz,
                         const double* pscore, const int32_t* nevals, int32_t num_modes,
                         double min_rmsd, int32_t* out_index);
 
+/* ---- Monte-Carlo walker (searcher.cpp:46-223) ----
+ * Walker `walker` of ligand `ligand` draws Philox4x32-10 blocks with key
+ * (seed_lo, seed_hi + walker) and counter (draw, ligand, 0, 0); walker 0
+ * optimises xyz0 and takes no Monte-Carlo step.  best_tors has n_leaves
+ * entries.  Returns the number of Monte-Carlo steps taken. */
+int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
+                     const pxd_mc_params* prm, const pxd_box* box, int32_t walker,
+                     int32_t ligand, const double* xyz0, double* best_xyz, double* best_tors,
+                     double* best_e, double* best_ps, int32_t* nevals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,7 @@ def lib():
                                      C.POINTER(C.c_uint32)]),
             "or_pose_cluster": (i, [i, i, f64p, f64p, i32p, i, d, i32p]),
             "or_set_eval_noise": (None, [d, C.c_uint64]),
+            "or_mc_walker": (i, [P, P, P, P, P, i, i, f64p, f64p, f64p, f64p, f64p, i32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -258,3 +259,18 @@ def receptor_apply(leaves, xyz, tors):
     out = np.zeros_like(x)
     lib().or_receptor_apply(byref(lc), len(x), dp(x), dp(f64(tors)), dp(out))
     return out
+
+
+def mc_walker(system, ligand, sp, prm, box, walker, ligand_index, xyz0):
+    """One Monte-Carlo walker (searcher.cpp:98-187) with the GPU's Philox draws."""
+    tc = ligand.tree.to_c()
+    n = ligand.n_atoms
+    bx = np.zeros((n, 3))
+    bt = np.zeros(max(system.n_leaves, 1))
+    be, bp = np.zeros(1), np.zeros(1)
+    ne = np.zeros(1, np.int32)
+    steps = lib().or_mc_walker(byref(system.c), byref(tc), byref(sp), byref(prm), byref(box),
+                               walker, ligand_index, dp(f64(xyz0).reshape(-1, 3)), dp(bx),
+                               dp(bt), dp(be), dp(bp), ip(ne))
+    return dict(xyz=bx, tors=bt[:system.n_leaves], energy=float(be[0]), pscore=float(bp[0]),
+                nevals=int(ne[0]), steps=int(steps))

@@ -5,6 +5,7 @@
 #include "pxd_device.hpp"
 #include "pxd_kernels.hpp"
 #include "pxd_pscore.hpp"
+#include "pxd_mc.hpp"
 
 namespace pxd {
 
@@ -641,6 +642,376 @@ void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_pa
   if (a.n_poses <= 0) return;
   hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
                      a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms);
+}
+
+// ============================================================== K4: Monte-Carlo search
+// monte_carlo_searcher::mutate_and_optimize (searcher.cpp:98-187) for many
+// walkers, one wavefront per walker from a device work queue.  Walker state
+// (current and best conformations, step, RNG counter) stays in HBM between
+// launches; a launch advances every walker to step_end.  Each iteration of
+// the walker loop makes exactly one L-BFGS run and one pscore evaluation,
+// selected by the walker's phase (init / relax / refine).
+
+enum { PH_INIT = 0, PH_MUTATE = 1, PH_REFINE = 2 };
+
+template <int VPL>
+__global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
+                                           const DevGrid* __restrict__ G, McArgs A) {
+  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ float vpool[V_COUNT * kWave * VPL];
+  __shared__ double s_ro[kMaxHistory];
+  __shared__ double s_al[kMaxHistory];
+  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
+  __shared__ double s_terms[PXD_NUM_TERMS];
+  __shared__ float apl[PXD_APPLY_LDS_FLOATS];
+  __shared__ float curx[kMaxAtoms * 3 + kMaxLeaves];
+  __shared__ int s_w;
+  const int lane = lane_id();
+  constexpr int NV = kWave * VPL;
+  const McParamsDev& P = A.prm;
+  const pxd_score_params& sp = *A.sp;
+  const int nl = R.n_leaves;
+  float* hy = A.hist + (size_t)blockIdx.x * 2 * A.hist_m * NV;
+  float* hs = hy + (size_t)A.hist_m * NV;
+  LbfgsCtx<VPL> c;
+  c.R = &R;
+  c.L = &L;
+  c.G = G;
+  c.s = carve_pose_lds(lds);
+  c.v = vpool;
+  PscoreLds ps;
+  ps.rc = rings;
+  ps.rn = rings + 3 * kMaxRecRings;
+  ps.lc = rings + 6 * kMaxRecRings;
+  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps.terms = s_terms;
+  ApplyLds ap = carve_apply_lds(apl);
+  BoxF box;
+  for (int d = 0; d < 3; ++d) {
+    box.lo[d] = A.box_lo[d];
+    box.hi[d] = A.box_hi[d];
+  }
+  float* x = c.vec(V_X);
+  float* curt = curx + kMaxAtoms * 3;
+  for (;;) {
+    if (lane == 0) s_w = atomicAdd(A.counter, 1);
+    __syncthreads();
+    const int w = s_w;
+    __syncthreads();
+    if (w >= A.n_walkers) break;
+    WalkerState st = A.st[w];
+    if (st.mode == 2) continue;
+    const int pid = A.walker_pid[w];
+    c.li = L.info[A.walker_lig[w]];
+    const LigInfo& li = c.li;
+    const int n = li.n_atoms, n3 = 3 * n;
+    c.n3 = n3;
+    c.n = n3 + nl;
+    load_ligand_params(L, li, G, c.s);
+    Rng rng{P.seed_lo, P.seed_hi + (uint32_t)pid, (uint32_t)A.walker_lig[w], st.rng_ctr};
+    const long long off = 3ll * A.walker_off[w];
+    for (int k = lane; k < n3; k += kWave) curx[k] = A.cur_xyz[off + k];
+    for (int k = lane; k < nl; k += kWave) curt[k] = A.cur_tors[(long long)w * nl + k];
+    int phase;
+    if (st.mode == 0) {
+      // initial conformer (searcher.cpp:105-115): walker 0 keeps pose 0,
+      // the others are randomised (searcher.cpp:46-87)
+      for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = A.lig_xyz0[3ll * li.atom_off + k];
+      __syncthreads();
+      if (pid != 0) {
+        for (int tries = 1;; ++tries) {
+          float cen[3], rot[9];
+          cen[0] = (float)rng.uniform(box.lo[0], box.hi[0]);
+          cen[1] = (float)rng.uniform(box.lo[1], box.hi[1]);
+          cen[2] = (float)rng.uniform(box.lo[2], box.hi[2]);
+          double q[4];
+          rng.quaternion(q);
+          for (int f = lane; f < li.n_frags; f += kWave) ap.th[f] = 0.f;
+          __syncthreads();
+          // torsion draws are wave-uniform: every lane draws, lane 0 stores
+          for (int t = 0; t < li.n_tors; ++t) {
+            float v = (float)rng.uniform(-3.141592653589793, 3.141592653589793);
+            if (lane == 0) ap.th[L.tors_frag[li.frag_off + t]] = v;
+          }
+          quat_to_rot(q, rot);
+          for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = A.lig_xyz0[3ll * li.atom_off + k];
+          __syncthreads();
+          apply_ligand(L, li, c.s.xyz, cen, rot, ap);
+          bool inbox = all_in_box(c.s.xyz, n, box);
+          if (tries < P.max_tries && !inbox && tries < 4 * P.max_tries) continue;
+          float cc[3];
+          centroid(c.s.xyz, n, cc);
+          if (!point_in_box(cc, box) && tries < 4 * P.max_tries) continue;
+          if (tries >= P.max_tries) break;
+          if (pscore_vdw_initial(R, L, li, sp, c.s) < 100.0) break;
+        }
+      }
+      for (int k = lane; k < n3; k += kWave) x[k] = c.s.xyz[k];
+      for (int k = lane; k < nl; k += kWave) x[n3 + k] = 0.f;
+      phase = PH_INIT;
+    } else {
+      phase = PH_MUTATE;
+    }
+    const int step_end = A.step_end < P.max_nsteps ? A.step_end : P.max_nsteps;
+    for (;;) {
+      if (phase == PH_MUTATE) {
+        if (st.step >= step_end) break;
+        // mutate_inplace (searcher.cpp:189-223) on a copy of the current state
+        for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = curx[k];
+        for (int f = lane; f < li.n_frags; f += kWave) ap.th[f] = 0.f;
+        __syncthreads();
+        float cen[3], rot[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
+        centroid(c.s.xyz, n, cen);
+        bool apply = true;
+        if (P.mutation == 0) {
+          int which = rng.uniform_int(0, li.n_tors + 1);
+          if (which == 0) {
+            double dl[3];
+            rng.in_sphere(dl);
+            for (int d = 0; d < 3; ++d) cen[d] += (float)(dl[d] * P.amplitude);
+          } else if (which == 1) {
+            float gr = 0.f;
+            for (int a = lane; a < n; a += kWave)
+              for (int d = 0; d < 3; ++d) {
+                float v = c.s.xyz[3 * a + d] - cen[d];
+                gr += v * v;
+              }
+            gr = sqrtf(wave_sum(gr) / (float)n);
+            if (gr < 1e-6f) {
+              apply = false;  // searcher.cpp:207: mutation skipped
+            } else {
+              double dl[3];
+              rng.in_sphere(dl);
+              double sc = P.amplitude / gr;
+              double v[3] = {dl[0] * sc, dl[1] * sc, dl[2] * sc};
+              double ang = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+              rot_axis((float)(v[0] / ang), (float)(v[1] / ang), (float)(v[2] / ang), (float)ang, rot);
+            }
+          } else {
+            float v = (float)rng.uniform(-3.141592653589793, 3.141592653589793);
+            if (lane == 0) ap.th[L.tors_frag[li.frag_off + which - 2]] = v;
+          }
+        } else {
+          for (int d = 0; d < 3; ++d) cen[d] += (float)(rng.normal() * P.trans_sigma);
+          double v[3] = {rng.normal() * P.rot_sigma, rng.normal() * P.rot_sigma,
+                         rng.normal() * P.rot_sigma};
+          double ang = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          if (ang > 1e-12)
+            rot_axis((float)(v[0] / ang), (float)(v[1] / ang), (float)(v[2] / ang), (float)ang, rot);
+          if (li.n_tors > 0) {
+            int t = rng.uniform_int(0, li.n_tors - 1);
+            float th = (float)rng.uniform(-3.141592653589793, 3.141592653589793);
+            if (lane == 0) ap.th[L.tors_frag[li.frag_off + t]] = th;
+          }
+        }
+        __syncthreads();
+        if (apply) apply_ligand(L, li, c.s.xyz, cen, rot, ap);
+        for (int k = lane; k < n3; k += kWave) x[k] = c.s.xyz[k];
+        for (int k = lane; k < nl; k += kWave) x[n3 + k] = curt[k];
+      } else if (phase == PH_REFINE) {
+        for (int k = lane; k < n3; k += kWave) x[k] = curx[k];
+        for (int k = lane; k < nl; k += kWave) x[n3 + k] = curt[k];
+      }
+      __syncthreads();
+      // the single L-BFGS site: full step for init / refine, relax after a mutation
+      c.nevals = 0;
+      double e;
+      bool conv;
+      lbfgs_solve<VPL>(c, phase == PH_MUTATE ? P.relax : P.full, hy, hs, s_ro, s_al, &e, &conv);
+      // the single pscore site, at the optimised pose
+      for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = x[k];
+      for (int k = lane; k < nl; k += kWave) c.s.tors[k] = x[n3 + k];
+      __syncthreads();
+      receptor_torsion2xyz(R, c.s);
+      __syncthreads();
+      double psv = pscore_pose(R, L, li, sp, c.s, ps);
+      if (phase == PH_INIT) {
+        st.nevals = c.nevals;
+        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+        st.cur_e = st.best_e = e;
+        st.cur_ps = st.best_ps = psv;
+        for (int k = lane; k < n3; k += kWave) A.best_xyz[off + k] = x[k];
+        for (int k = lane; k < nl; k += kWave) A.best_tors[(long long)w * nl + k] = x[n3 + k];
+        __syncthreads();
+        if (pid == 0) {  // searcher.cpp:123: walker 0 does no Monte-Carlo
+          st.mode = 2;
+          break;
+        }
+        if (!all_in_box(curx, n, box)) {  // searcher.cpp:178-185
+          if (!coords_sane(curx, n)) st.nevals = 0;
+          st.mode = 2;
+          break;
+        }
+        st.mode = 1;
+        st.step = 0;
+        phase = PH_MUTATE;
+      } else if (phase == PH_MUTATE) {
+        st.nevals += c.nevals;
+        // Metropolis on pscore (searcher.cpp:164-165, searcher.h:53-56)
+        bool acc = st.step == 0;
+        if (!acc) {
+          if (psv < st.cur_ps)
+            acc = true;
+          else
+            acc = rng.u01() < exp((st.cur_ps - psv) / (double)P.temperature);
+        }
+        bool inbox = all_in_box(x, n, box);
+        if (acc && inbox) {
+          for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+          st.cur_e = e;
+          st.cur_ps = psv;
+          __syncthreads();
+          if (psv < st.best_ps) {
+            phase = PH_REFINE;
+            continue;
+          }
+        }
+        st.step++;
+      } else {  // PH_REFINE (searcher.cpp:167-174)
+        st.nevals += c.nevals;
+        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+        st.cur_e = e;
+        st.cur_ps = psv;
+        if (psv < st.best_ps) {
+          st.best_ps = psv;
+          st.best_e = e;
+          for (int k = lane; k < n3; k += kWave) A.best_xyz[off + k] = x[k];
+          for (int k = lane; k < nl; k += kWave) A.best_tors[(long long)w * nl + k] = x[n3 + k];
+        }
+        __syncthreads();
+        st.step++;
+        phase = PH_MUTATE;
+      }
+    }
+    if (st.mode == 1 && st.step >= P.max_nsteps) st.mode = 2;
+    st.rng_ctr = rng.ctr;
+    for (int k = lane; k < n3; k += kWave) A.cur_xyz[off + k] = curx[k];
+    for (int k = lane; k < nl; k += kWave) A.cur_tors[(long long)w * nl + k] = curt[k];
+    if (lane == 0) A.st[w] = st;
+    __syncthreads();
+  }
+}
+
+void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
+               int n_slots, hipStream_t st) {
+  hipLaunchKernelGGL(k_mc<kLbfgsVpl>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_int(int* p, int n, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+// ============================================================== K6: top-k clustering
+// pose_cluster::preprocess (writer.cpp:124-177) per ligand, one wave per ligand;
+// RMSDs (writer.cpp:99-106) reduced across lanes.
+
+__global__ __launch_bounds__(64) void k_cluster(ClusterArgs A) {
+  const int l = blockIdx.x;
+  if (l >= A.n_ligands) return;
+  const int lane = lane_id();
+  __shared__ int kept[64];
+  int cnt = 0;
+  const int w0 = A.lig_walker_off[l], w1 = A.lig_walker_off[l + 1];
+  for (int w = w0; w < w1; ++w) {
+    const WalkerState st = A.st[w];
+    if (st.nevals == 0 || st.mode == 0) continue;  // failure mark (writer.cpp:135)
+    if (cnt == 0) {
+      if (lane == 0) kept[0] = w;
+      cnt = 1;
+      __syncthreads();
+      continue;
+    }
+    int best = 0;
+    float rbest = 0.f;
+    const float* xw = A.best_xyz + 3ll * A.walker_off[w];
+    for (int k = 0; k < cnt; ++k) {
+      const float* xk = A.best_xyz + 3ll * A.walker_off[kept[k]];
+      int na = A.walker_off[w + 1] - A.walker_off[w];
+      float acc = 0.f;
+      for (int i = lane; i < 3 * na; i += kWave) {
+        float dv = xk[i] - xw[i];
+        acc += dv * dv;
+      }
+      float r = sqrtf(wave_sum(acc) / (float)na);
+      if (k == 0 || r < rbest) {
+        best = k;
+        rbest = r;
+      }
+    }
+    bool updated = false;
+    double ps = st.best_ps;
+    if (rbest < A.min_rmsd) {
+      if (ps < A.st[kept[best]].best_ps) {
+        if (lane == 0) kept[best] = w;
+        updated = true;
+      }
+    } else if (cnt < A.num_modes) {
+      if (lane == 0) kept[cnt] = w;
+      cnt++;
+      updated = true;
+    } else if (ps < A.st[kept[cnt - 1]].best_ps) {
+      if (lane == 0) kept[cnt - 1] = w;
+      updated = true;
+    }
+    __syncthreads();
+    if (updated && lane == 0) {  // stable insertion sort by pscore
+      for (int a = 1; a < cnt; ++a) {
+        int v = kept[a], b = a - 1;
+        while (b >= 0 && A.st[kept[b]].best_ps > A.st[v].best_ps) {
+          kept[b + 1] = kept[b];
+          --b;
+        }
+        kept[b + 1] = v;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = lane; k < A.num_modes; k += kWave)
+    A.out_idx[l * A.num_modes + k] = k < cnt ? kept[k] - w0 : -1;
+  if (lane == 0) A.out_count[l] = cnt;
+}
+
+void launch_cluster(const ClusterArgs& a, int n_atoms_max, hipStream_t st) {
+  (void)n_atoms_max;
+  hipLaunchKernelGGL(k_cluster, dim3(a.n_ligands), dim3(kWave), 0, st, a);
+}
+
+// ============================================================== pose parameterisation
+// free_ligand::apply_parameters for a batch of poses (one wave per pose):
+// new centre, orientation matrix and torsion rotations applied to the input
+// conformation of each pose.
+
+__global__ __launch_bounds__(64) void k_apply(DevLigands L, ApplyArgs A) {
+  __shared__ float xyz[kMaxAtoms * 3];
+  __shared__ float apl[PXD_APPLY_LDS_FLOATS];
+  const int p = blockIdx.x;
+  if (p >= A.n_poses) return;
+  const int lane = lane_id();
+  const LigInfo li = L.info[A.pose_lig[p]];
+  ApplyLds ap = carve_apply_lds(apl);
+  const long long off = 3ll * A.pose_off[p];
+  for (int k = lane; k < 3 * li.n_atoms; k += kWave) xyz[k] = A.xyz_in[off + k];
+  for (int f = lane; f < li.n_frags; f += kWave) ap.th[f] = 0.f;
+  __syncthreads();
+  const int t0 = A.tors_off[p], t1 = A.tors_off[p + 1];
+  for (int t = lane; t < t1 - t0 && t < li.n_tors; t += kWave)
+    ap.th[L.tors_frag[li.frag_off + t]] = A.tors[t0 + t];
+  float cen[3] = {A.center[3 * p], A.center[3 * p + 1], A.center[3 * p + 2]};
+  float rot[9];
+  for (int k = 0; k < 9; ++k) rot[k] = A.rot[9 * p + k];
+  __syncthreads();
+  apply_ligand(L, li, xyz, cen, rot, ap);
+  for (int k = lane; k < 3 * li.n_atoms; k += kWave) A.xyz_out[off + k] = xyz[k];
+}
+
+void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st) {
+  if (a.n_poses <= 0) return;
+  hipLaunchKernelGGL(k_apply, dim3(a.n_poses), dim3(kWave), 0, st, L, a);
 }
 
 }  // namespace pxd

@@ -63,6 +63,57 @@ struct PscoreArgs {
   double* terms;
 };
 
+struct McParamsDev {
+  uint32_t seed_lo, seed_hi;
+  int max_nsteps;
+  int mutation;        // 0: reference mutate_inplace, 1: Gaussian (translation + rotation + torsion)
+  float amplitude, temperature, trans_sigma, rot_sigma;
+  int max_tries;
+  LbfgsParamsDev relax, full;
+};
+
+// Per-walker scalars kept on the device between launches.
+struct WalkerState {
+  int mode;       // 0: needs init, 1: stepping, 2: done
+  int step;
+  uint32_t rng_ctr;
+  int nevals;
+  double cur_e, cur_ps, best_e, best_ps;
+};
+
+struct McArgs {
+  McParamsDev prm;
+  const pxd_score_params* sp;  // device copy
+  float box_lo[3], box_hi[3];
+  int n_walkers;
+  const int* walker_lig;
+  const int* walker_pid;   // walker index within its ligand (0: no Monte-Carlo)
+  const int* walker_off;   // atom offset of the walker's conformations
+  const float* lig_xyz0;   // input conformation, per ligand atom offset
+  float* cur_xyz;
+  float* best_xyz;
+  float* cur_tors;
+  float* best_tors;
+  WalkerState* st;
+  int* counter;
+  float* hist;
+  int hist_m;      // history slots per wave in hist
+  int step_end;
+};
+
+struct ClusterArgs {
+  int n_ligands;
+  const int* lig_walker_off;  // [n_ligands+1] walkers of ligand l
+  const int* walker_off;
+  const int* walker_lig;
+  const float* best_xyz;
+  const WalkerState* st;
+  int num_modes;
+  float min_rmsd;
+  int* out_idx;    // [n_ligands * num_modes] walker indices (-1 padded)
+  int* out_count;  // [n_ligands]
+};
+
 void launch_grid_build(const GridBuildArgs& a, hipStream_t st);
 void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const ScoreArgs& a, hipStream_t st);
@@ -70,5 +121,21 @@ void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st);
 void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                    const PscoreArgs& a, hipStream_t st);
+void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
+               int n_slots, hipStream_t st);
+void launch_fill_int(int* p, int n, int v, hipStream_t st);
+void launch_cluster(const ClusterArgs& a, int n_atoms_max, hipStream_t st);
+struct ApplyArgs {
+  int n_poses;
+  const int* pose_lig;
+  const int* pose_off;
+  const float* xyz_in;
+  const float* center;   // [n][3]
+  const float* rot;      // [n][9]
+  const int* tors_off;   // [n+1] into tors
+  const float* tors;
+  float* xyz_out;
+};
+void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st);
 
 }  // namespace pxd

@@ -1,0 +1,252 @@
+// pxd_search.cpp — C ABI of the device Monte-Carlo search, top-k clustering
+// and batched pose parameterisation.  The step loop (one k_mc launch per
+// segment of steps, then clustering) is captured once into a hipGraph on a
+// private stream and replayed; walker state never leaves the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "pxd_internal.hpp"
+#include "pxd_kernels.hpp"
+#include "pxdock.h"
+
+using namespace pxd;
+
+extern "C" pxd_status pxd_set_error_(pxd_status st, const char* msg);
+
+namespace {
+
+LbfgsParamsDev to_dev(const pxd_lbfgs_params& p) {
+  return {p.history, p.max_iter, p.max_eval, p.max_ls, p.lr, p.tol_grad,
+          p.tol_change, p.c1, p.c2, p.ls_tol_change, p.gtd_threshold};
+}
+
+bool bad_lbfgs(const pxd_lbfgs_params& p) {
+  return p.history < 1 || p.history > kMaxHistory || p.max_iter < 0 || p.max_eval < 0 ||
+         p.max_ls < 0;
+}
+
+#define HIP_OK(expr)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) {                                            \
+      err = e_;                                                        \
+      goto done;                                                       \
+    }                                                                  \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+void pxd_mc_default_params(pxd_mc_params* p) {
+  memset(p, 0, sizeof(*p));
+  p->seed = 0;
+  p->max_nsteps = 200;
+  p->mutation = 0;
+  p->amplitude = 2.0;
+  p->temperature = 1.2;
+  p->trans_sigma = 1.0;
+  p->rot_sigma = 0.3;
+  p->max_tries = 2000;
+  p->steps_per_launch = 10;
+  pxd_lbfgs_default_params(&p->relax, 40);
+  pxd_lbfgs_default_params(&p->full, 240);
+}
+
+pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                         const pxd_score_params* sp, const pxd_mc_params* prm, const pxd_box* box,
+                         int32_t walkers, const float* lig_xyz0, float* best_xyz,
+                         float* best_tors, double* best_energy, double* best_pscore,
+                         int32_t* nevals, int32_t num_modes, double min_rmsd,
+                         int32_t* cluster_idx, int32_t* cluster_count, void* stream) {
+  if (!r || !b || !sp || !prm || !box || !lig_xyz0 || !best_xyz || !best_energy ||
+      !best_pscore || !nevals)
+    return pxd_set_error_(PXD_ERR_ARG, "mc_search: null argument");
+  if (walkers < 1) return pxd_set_error_(PXD_ERR_ARG, "mc_search: walkers < 1");
+  if (r->n_leaves > 0 && !best_tors) return pxd_set_error_(PXD_ERR_ARG, "mc_search: best_tors required");
+  if (bad_lbfgs(prm->relax) || bad_lbfgs(prm->full))
+    return pxd_set_error_(PXD_ERR_RANGE, "mc_search: L-BFGS parameters out of range");
+  if (prm->max_nsteps < 0 || prm->max_tries < 1 || prm->steps_per_launch < 1)
+    return pxd_set_error_(PXD_ERR_ARG, "mc_search: bad step parameters");
+  if (num_modes < 0 || num_modes > 64)
+    return pxd_set_error_(PXD_ERR_RANGE, "mc_search: num_modes outside 0..64");
+  if (num_modes > 0 && (!cluster_idx || !cluster_count))
+    return pxd_set_error_(PXD_ERR_ARG, "mc_search: cluster outputs required");
+  if (r->dev.n_ring5 + r->dev.n_ring6 > 256)
+    return pxd_set_error_(PXD_ERR_RANGE, "mc_search: receptor has more than 256 pi rings");
+  if (g) {
+    for (int t = 0; t < PXD_NUM_VDW_TYPES; ++t)
+      if (b->type_used[t] && g->dev.type_slot[t] < 0)
+        return pxd_set_error_(PXD_ERR_STATE, "mc_search: ligand type without a map");
+  }
+  for (int l = 0; l < b->n; ++l)
+    if (b->h_ntors[l] + 1 > PXD_MAX_FRAGMENTS)
+      return pxd_set_error_(PXD_ERR_RANGE, "mc_search: too many torsions");
+
+  const int nl = b->n, W = nl * walkers, L = r->n_leaves;
+  std::vector<int> wlig(W), wpid(W), woff(W + 1), lwoff(nl + 1);
+  long long atoms = 0;
+  for (int l = 0, w = 0; l < nl; ++l) {
+    lwoff[l] = w;
+    for (int k = 0; k < walkers; ++k, ++w) {
+      wlig[w] = l;
+      wpid[w] = k;
+      woff[w] = (int)atoms;
+      atoms += b->h_natoms[l];
+    }
+  }
+  lwoff[nl] = W;
+  woff[W] = (int)atoms;
+  if (atoms > (1ll << 30)) return pxd_set_error_(PXD_ERR_RANGE, "mc_search: too many atoms");
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return pxd_set_error_(PXD_ERR_HIP, "mc_search: no current HIP device");
+  const int n_slots = std::min(W, std::max(1, ncu) * 4);
+  const int hist_m = std::max(prm->relax.history, prm->full.history);
+  const int n_launch = (prm->max_nsteps + prm->steps_per_launch - 1) / prm->steps_per_launch + 1;
+  const size_t hist_floats = (size_t)n_slots * 2 * hist_m * kWave * kLbfgsVpl;
+  hipError_t err = hipSuccess;
+  hipStream_t user = static_cast<hipStream_t>(stream), cs = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  char* mem = nullptr;
+  pxd_score_params* d_sp = nullptr;
+  int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt;
+  float *d_cur, *d_curt, *d_hist;
+  WalkerState* d_st;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t o_sp = take(sizeof(pxd_score_params)), o_wlig = take(4 * W), o_wpid = take(4 * W),
+               o_woff = take(4 * (W + 1)), o_lwoff = take(4 * (nl + 1)),
+               o_cnt = take(4 * (size_t)n_launch), o_cur = take(12 * (size_t)atoms),
+               o_curt = take(4 * (size_t)W * (L + 1)), o_st = take(sizeof(WalkerState) * W),
+               o_hist = take(4 * hist_floats);
+  HIP_OK(hipMalloc(&mem, off));
+  d_sp = reinterpret_cast<pxd_score_params*>(mem + o_sp);
+  d_wlig = reinterpret_cast<int*>(mem + o_wlig);
+  d_wpid = reinterpret_cast<int*>(mem + o_wpid);
+  d_woff = reinterpret_cast<int*>(mem + o_woff);
+  d_lwoff = reinterpret_cast<int*>(mem + o_lwoff);
+  d_cnt = reinterpret_cast<int*>(mem + o_cnt);
+  d_cur = reinterpret_cast<float*>(mem + o_cur);
+  d_curt = reinterpret_cast<float*>(mem + o_curt);
+  d_st = reinterpret_cast<WalkerState*>(mem + o_st);
+  d_hist = reinterpret_cast<float*>(mem + o_hist);
+  HIP_OK(hipMemcpy(d_sp, sp, sizeof(*sp), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_wlig, wlig.data(), 4 * W, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_wpid, wpid.data(), 4 * W, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_woff, woff.data(), 4 * (W + 1), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_lwoff, lwoff.data(), 4 * (nl + 1), hipMemcpyHostToDevice));
+  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  HIP_OK(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  HIP_OK(hipEventRecord(ev0, user));
+  HIP_OK(hipStreamWaitEvent(cs, ev0, 0));
+  {
+    McArgs a;
+    memset(&a, 0, sizeof(a));
+    a.prm.seed_lo = (uint32_t)prm->seed;
+    a.prm.seed_hi = (uint32_t)(prm->seed >> 32);
+    a.prm.max_nsteps = prm->max_nsteps;
+    a.prm.mutation = prm->mutation;
+    a.prm.amplitude = (float)prm->amplitude;
+    a.prm.temperature = (float)prm->temperature;
+    a.prm.trans_sigma = (float)prm->trans_sigma;
+    a.prm.rot_sigma = (float)prm->rot_sigma;
+    a.prm.max_tries = prm->max_tries;
+    a.prm.relax = to_dev(prm->relax);
+    a.prm.full = to_dev(prm->full);
+    a.sp = d_sp;
+    for (int d = 0; d < 3; ++d) {
+      a.box_lo[d] = (float)box->init_xyz[d];
+      a.box_hi[d] = (float)box->oppo_xyz[d];
+    }
+    a.n_walkers = W;
+    a.walker_lig = d_wlig;
+    a.walker_pid = d_wpid;
+    a.walker_off = d_woff;
+    a.lig_xyz0 = lig_xyz0;
+    a.cur_xyz = d_cur;
+    a.best_xyz = best_xyz;
+    a.cur_tors = d_curt;
+    a.best_tors = best_tors ? best_tors : d_curt;
+    a.st = d_st;
+    a.hist = d_hist;
+    a.hist_m = hist_m;
+    // capture: reset state and queue heads with fill kernels (no memset nodes),
+    // then one k_mc launch per segment of steps, then clustering
+    HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
+    launch_fill_int(d_cnt, n_launch, 0, cs);
+    for (int k = 0; k < n_launch; ++k) {
+      a.counter = d_cnt + k;
+      a.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
+      launch_mc(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, a, n_slots, cs);
+    }
+    if (num_modes > 0) {
+      ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
+                     cluster_idx, cluster_count};
+      launch_cluster(ca, 0, cs);
+    }
+    err = hipStreamEndCapture(cs, &graph);
+    if (err != hipSuccess) goto done;
+  }
+  HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIP_OK(hipGraphLaunch(exec, cs));
+  HIP_OK(hipStreamSynchronize(cs));
+  {
+    // per-walker scalars out of the walker state
+    std::vector<WalkerState> st(W);
+    HIP_OK(hipMemcpy(st.data(), d_st, sizeof(WalkerState) * W, hipMemcpyDeviceToHost));
+    std::vector<double> be(W), bp(W);
+    std::vector<int> ne(W);
+    for (int w = 0; w < W; ++w) {
+      be[w] = st[w].best_e;
+      bp[w] = st[w].best_ps;
+      ne[w] = st[w].nevals;
+    }
+    HIP_OK(hipMemcpy(best_energy, be.data(), 8 * W, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(best_pscore, bp.data(), 8 * W, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(nevals, ne.data(), 4 * W, hipMemcpyHostToDevice));
+  }
+  HIP_OK(hipEventRecord(ev1, cs));
+  HIP_OK(hipStreamWaitEvent(user, ev1, 0));
+done:
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (cs) {
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+  }
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (mem) (void)hipFree(mem);
+  if (err != hipSuccess) return pxd_set_error_(PXD_ERR_HIP, hipGetErrorString(err));
+  return PXD_OK;
+}
+
+pxd_status pxd_apply_poses(const pxd_ligand_batch* b, int32_t n_poses, const int32_t* pose_lig,
+                           const int32_t* pose_atom_off, const float* xyz_in, const float* center,
+                           const float* rot, const int32_t* tors_off, const float* tors,
+                           float* xyz_out, void* stream) {
+  if (!b || n_poses < 0) return pxd_set_error_(PXD_ERR_ARG, "apply_poses: bad argument");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz_in || !center || !rot || !tors_off || !xyz_out)
+    return pxd_set_error_(PXD_ERR_ARG, "apply_poses: null array");
+  ApplyArgs a{n_poses, pose_lig, pose_atom_off, xyz_in, center, rot, tors_off, tors, xyz_out};
+  launch_apply(b->dev, a, static_cast<hipStream_t>(stream));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return pxd_set_error_(PXD_ERR_HIP, hipGetErrorString(e));
+  return PXD_OK;
+}
+
+}  // extern "C"

@@ -126,6 +126,32 @@ def lbfgs_params(max_iter, history=100, max_eval=None, tol_grad=0.5, tol_change=
     return p
 
 
+class PxdMcParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("max_nsteps", C.c_int32), ("mutation", C.c_int32),
+        ("amplitude", C.c_double), ("temperature", C.c_double), ("trans_sigma", C.c_double),
+        ("rot_sigma", C.c_double), ("max_tries", C.c_int32), ("steps_per_launch", C.c_int32),
+        ("relax", PxdLbfgsParams), ("full", PxdLbfgsParams),
+    ]
+
+
+def mc_params(seed=0, max_nsteps=200, relax_iters=40, full_iters=240, steps_per_launch=10):
+    """monte_carlo_searcher defaults (searcher.h:33, bdock.cpp:60-78)."""
+    p = PxdMcParams()
+    p.seed = seed
+    p.max_nsteps = max_nsteps
+    p.mutation = 0
+    p.amplitude = 2.0
+    p.temperature = 1.2
+    p.trans_sigma = 1.0
+    p.rot_sigma = 0.3
+    p.max_tries = 2000
+    p.steps_per_launch = steps_per_launch
+    p.relax = lbfgs_params(relax_iters)
+    p.full = lbfgs_params(full_iters)
+    return p
+
+
 def ptr(arr, ctype):
     """Pointer to a contiguous numpy array (None for None/empty)."""
     if arr is None:

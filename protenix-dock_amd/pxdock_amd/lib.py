@@ -62,6 +62,9 @@ def load():
         "pxd_lbfgs_batch": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_pscore_batch": (i, [P, P, P, i, P, P, P, P, P, P, P]),
         "pxd_lbfgs_plain": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_mc_default_params": (None, [P]),
+        "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
+        "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -268,3 +271,81 @@ def pscore(receptor, poses, params, with_terms=True):
                                   _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
                                   _dev(poses.tors), _dev(s), _dev(t), _stream()))
     return s, t
+
+
+def mc_default_params():
+    """pxd_mc_default_params: the reference's search defaults."""
+    p = ffi.PxdMcParams()
+    load().pxd_mc_default_params(_ref(p))
+    return p
+
+
+class SearchResult:
+    """Per-walker outputs of pxd_mc_search (device tensors) and the clustering."""
+
+    def __init__(self, batch, walkers, best_xyz, best_tors, best_energy, best_pscore, nevals,
+                 cluster_idx, cluster_count):
+        self.batch = batch
+        self.walkers = walkers
+        self.best_xyz = best_xyz
+        self.best_tors = best_tors
+        self.best_energy = best_energy
+        self.best_pscore = best_pscore
+        self.nevals = nevals
+        self.cluster_idx = cluster_idx
+        self.cluster_count = cluster_count
+        n = batch.natoms
+        self.walker_atom_off = np.concatenate([[0], np.cumsum(np.repeat(n, walkers))[:-1]]) \
+            .astype(np.int32)
+
+    def walker_xyz(self, w, arr=None):
+        a = (self.best_xyz if arr is None else arr).detach().cpu().numpy().reshape(-1, 3)
+        o = self.walker_atom_off[w]
+        return a[o:o + self.batch.natoms[w // self.walkers]]
+
+
+def mc_search(receptor, grid, batch, lig_xyz0, score_params, params, box, walkers,
+              num_modes=0, min_rmsd=1.0):
+    """pxd_mc_search: `walkers` Monte-Carlo walkers per ligand of `batch`.
+    lig_xyz0: list of [natoms, 3] input conformations, one per ligand."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in lig_xyz0])
+    assert len(flat) == int(batch.natoms.sum()), "input conformations do not match the batch"
+    x0 = torch.from_numpy(flat.astype(np.float32).ravel()).to(dev)
+    W = len(batch.natoms) * walkers
+    nl = receptor.n_leaves
+    bx = torch.zeros(3 * int(batch.natoms.sum()) * walkers, dtype=torch.float32, device=dev)
+    bt = torch.zeros((W, max(nl, 1)), dtype=torch.float32, device=dev)
+    be = torch.zeros(W, dtype=torch.float64, device=dev)
+    bp = torch.zeros(W, dtype=torch.float64, device=dev)
+    ne = torch.zeros(W, dtype=torch.int32, device=dev)
+    ci = torch.full((len(batch.natoms), max(num_modes, 1)), -1, dtype=torch.int32, device=dev)
+    cc = torch.zeros(len(batch.natoms), dtype=torch.int32, device=dev)
+    check(load().pxd_mc_search(receptor.h, grid.h if grid is not None else None, batch.h,
+                               _ref(score_params), _ref(params), _ref(box), walkers, _dev(x0),
+                               _dev(bx), _dev(bt), _dev(be), _dev(bp), _dev(ne), num_modes,
+                               min_rmsd, _dev(ci), _dev(cc), _stream()))
+    return SearchResult(batch, walkers, bx, bt, be, bp, ne, ci, cc)
+
+
+def apply_poses(batch, pose_lig, xyz_in, center, rot, tors_list):
+    """pxd_apply_poses: rebuild each pose from its input conformation, a new
+    centre, a row-major orientation matrix and the ligand's torsion rotations."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    pose_lig = np.ascontiguousarray(pose_lig, dtype=np.int32)
+    n = batch.natoms[pose_lig]
+    off = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int32)
+    toff = np.concatenate([[0], np.cumsum([len(t) for t in tors_list])]).astype(np.int32)
+    flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in xyz_in])
+    tflat = np.concatenate([np.asarray(t, np.float64).ravel() for t in tors_list] + [np.zeros(1)])
+    T = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt).ravel()).to(dev)
+    d_lig, d_off, d_toff = T(pose_lig, np.int32), T(off, np.int32), T(toff, np.int32)
+    d_x, d_c, d_r, d_t = (T(flat, np.float32), T(center, np.float32), T(rot, np.float32),
+                          T(tflat, np.float32))
+    out = torch.empty_like(d_x)
+    check(load().pxd_apply_poses(batch.h, len(pose_lig), _dev(d_lig), _dev(d_off), _dev(d_x),
+                                 _dev(d_c), _dev(d_r), _dev(d_toff), _dev(d_t), _dev(out),
+                                 _stream()))
+    return out.cpu().numpy().reshape(-1, 3), off

@@ -1,0 +1,194 @@
+"""Device Monte-Carlo search, clustering and pose parameterisation (through
+the C ABI) against the FP64 oracle.
+
+The walk is driven by the same Philox4x32-10 stream on both sides, so:
+  * with the optimiser off (relax/full max_iter = 0) a walker's trajectory
+    is decided by its draws and by pscore comparisons only: GPU and oracle
+    walk the same path unless an FP32-vs-FP64 pscore lands within rounding
+    of a Metropolis or box threshold (allowed for at most 1 walker in 8);
+    poses agree to 2e-3 A, pscores to 1e-4 relative;
+  * walker 0 runs exactly one full L-BFGS: it equals the batched L-BFGS
+    entry point from the same start (same device code);
+  * with the optimiser on, walks are chaotic in FP32 vs FP64, so a full
+    search is checked through what it reports about itself: the best pose's
+    energy and pscore re-evaluated by the score and pscore entry points, the
+    box, the evaluation counts, and the clustering against the oracle's
+    pose_cluster run on the device results;
+  * splitting the step loop over more or fewer launches changes nothing.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pxdock_amd import ffi, lib, scoring, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sw():
+    import torch
+    assert torch.cuda.is_available(), "a GPU is required"
+    rec = synth.make_receptor(3, n_heavy=500, radius=16.0, pocket_offset=7.0, n_ring6=3,
+                              n_ring5=1, n_leaves=3)
+    ligs = [synth.make_ligand(44, n_heavy=16, n_torsions=4, n_rings=1),
+            synth.make_ligand(45, n_heavy=24, n_torsions=6, n_rings=2)]
+    box = lib.decide_grid_dims(rec.pocket_center, [16.0] * 3, 0.5)
+    types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
+    R = lib.Receptor(rec)
+    G = lib.Grid(R, box, types)
+    og = O.Grid(box, G.types, G.download().astype(np.float64), 1e6, rec.leaves.movable())
+    x0 = [np.asarray(np.asarray(synth.ligand_apply(m.tree, m.xyz[0], rec.pocket_center, np.eye(3),
+                                                   np.zeros(m.tree.n_torsions)), np.float32),
+                     np.float64) for m in ligs]
+    return dict(rec=rec, ligs=ligs, box=box, R=R, G=G, og=og, x0=x0,
+                sp=scoring.load_pose_selection(), B=lib.LigandBatch(ligs),
+                systems=[O.System(rec, m, og) for m in ligs])
+
+
+def _no_opt(seed, steps, spl=10):
+    prm = ffi.mc_params(seed=seed, max_nsteps=steps, steps_per_launch=spl)
+    prm.relax = ffi.lbfgs_params(0)
+    prm.full = ffi.lbfgs_params(0)
+    return prm
+
+
+def _in_box(x, box):
+    lo, hi = np.array(box.init_xyz), np.array(box.oppo_xyz)
+    return bool(((x > lo) & (x < hi)).all())
+
+
+def _compare_with_oracle(sw, prm, walkers):
+    print(f"\n[mc] seed={prm.seed} steps={prm.max_nsteps} walkers={walkers}", flush=True)
+    res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], walkers)
+    bp, be, ne = (res.best_pscore.cpu().numpy(), res.best_energy.cpu().numpy(),
+                  res.nevals.cpu().numpy())
+    bt = res.best_tors.cpu().numpy()
+    nl = sw["rec"].leaves.n_leaves
+    ok, rows = 0, []
+    for l, lig in enumerate(sw["ligs"]):
+        for k in range(walkers):
+            w = l * walkers + k
+            o = O.mc_walker(sw["systems"][l], lig, sw["sp"], prm, sw["box"], k, l, sw["x0"][l])
+            x = res.walker_xyz(w)
+            dx = float(np.abs(x - o["xyz"]).max())
+            same = (dx <= 2e-3 and abs(bp[w] - o["pscore"]) <= 1e-4 * max(abs(o["pscore"]), 1.0)
+                    and ne[w] == o["nevals"]
+                    and np.abs(bt[w, :nl] - o["tors"]).max(initial=0.0) <= 1e-4)
+            ok += same
+            rows.append((l, k, bp[w], o["pscore"], int(ne[w]), o["nevals"], dx))
+    for r in rows:
+        print("   ", r)
+    print(f"   agree {ok}/{len(rows)}")
+    return ok, len(rows), res
+
+
+def test_gpu_randomize_matches_oracle(sw):
+    ok, n, res = _compare_with_oracle(sw, _no_opt(11, 0), 8)
+    assert ok >= n - n // 8, (ok, n)
+    for w in range(n):
+        if w % 8:
+            assert _in_box(res.walker_xyz(w), sw["box"])
+
+
+def test_gpu_mc_walk_matches_oracle_without_optimiser(sw):
+    ok, n, _ = _compare_with_oracle(sw, _no_opt(5, 20), 8)
+    assert ok >= n - n // 8, (ok, n)
+
+
+def test_gpu_mc_launch_split_invariance(sw):
+    """The walk is a function of the draws only: 1, 3 or all steps per launch
+    give the same bits."""
+    outs = []
+    for spl in (1, 3, 20):
+        print(f"\n[mc] steps_per_launch={spl}", flush=True)
+        prm = ffi.mc_params(seed=2, max_nsteps=6, relax_iters=10, full_iters=30,
+                            steps_per_launch=spl)
+        r = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], 4)
+        outs.append((r.best_xyz.cpu().numpy(), r.best_pscore.cpu().numpy(),
+                     r.nevals.cpu().numpy()))
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o[0], outs[0][0])
+        np.testing.assert_array_equal(o[1], outs[0][1])
+        np.testing.assert_array_equal(o[2], outs[0][2])
+
+
+def test_gpu_walker0_equals_batched_lbfgs(sw):
+    prm = ffi.mc_params(seed=9, max_nsteps=5, relax_iters=10, full_iters=60)
+    print("\n[mc] walker 0 vs pxd_lbfgs_batch", flush=True)
+    res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], 1)
+    nl = sw["rec"].leaves.n_leaves
+    P = lib.PoseSet(sw["B"], [0, 1], sw["x0"], nl)
+    e, ne, _ = lib.lbfgs(sw["R"], sw["G"], P, prm.full)
+    e, ne = e.cpu().numpy(), ne.cpu().numpy()
+    np.testing.assert_allclose(res.best_energy.cpu().numpy(), e, rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(res.nevals.cpu().numpy(), ne)
+    for p in range(2):
+        np.testing.assert_allclose(res.walker_xyz(p), P.xyz_of(p), atol=1e-6)
+    # the pscore of walker 0 is the pscore entry point at that pose
+    s, _ = lib.pscore(sw["R"], P, sw["sp"], with_terms=False)
+    np.testing.assert_allclose(res.best_pscore.cpu().numpy(), s.cpu().numpy(), rtol=1e-9)
+
+
+def test_gpu_full_search_self_consistent_and_clustered(sw):
+    walkers, modes = 8, 4
+    prm = ffi.mc_params(seed=17, max_nsteps=12, relax_iters=20, full_iters=60)
+    print("\n[mc] full search + clustering", flush=True)
+    res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], walkers,
+                        num_modes=modes, min_rmsd=1.0)
+    bp, be, ne = (res.best_pscore.cpu().numpy(), res.best_energy.cpu().numpy(),
+                  res.nevals.cpu().numpy())
+    bt = res.best_tors.cpu().numpy()
+    nl = sw["rec"].leaves.n_leaves
+    W = len(sw["ligs"]) * walkers
+    pose_lig = [w // walkers for w in range(W)]
+    P = lib.PoseSet(sw["B"], pose_lig, [res.walker_xyz(w) for w in range(W)], nl, tors=bt)
+    s, _ = lib.pscore(sw["R"], P, sw["sp"], with_terms=False)
+    e, _, _ = lib.score_grad(sw["R"], sw["G"], P)
+    s, e = s.cpu().numpy(), e.cpu().numpy()
+    for w in range(W):
+        assert ne[w] > 0
+        assert abs(bp[w] - s[w]) <= 1e-6 * max(abs(s[w]), 1.0), (w, bp[w], s[w])
+        assert abs(be[w] - e[w]) <= 1e-5 * max(abs(e[w]), 1.0), (w, be[w], e[w])
+        if w % walkers:
+            assert _in_box(res.walker_xyz(w), sw["box"])
+    # walkers that moved never end above their first optimised pose
+    p0 = ffi.mc_params(seed=17, max_nsteps=0, relax_iters=20, full_iters=60)
+    first = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], p0, sw["box"], walkers)
+    assert (bp <= first.best_pscore.cpu().numpy() + 1e-12).all()
+    # clustering vs the oracle's pose_cluster on the same candidates
+    ci, cc = res.cluster_idx.cpu().numpy(), res.cluster_count.cpu().numpy()
+    for l, lig in enumerate(sw["ligs"]):
+        xs = np.stack([res.walker_xyz(l * walkers + k) for k in range(walkers)]).astype(np.float64)
+        out = np.zeros(modes, np.int32)
+        cnt = O.lib().or_pose_cluster(walkers, lig.n_atoms, O.dp(np.ascontiguousarray(xs)),
+                                      O.dp(np.ascontiguousarray(bp[l * walkers:(l + 1) * walkers])),
+                                      O.ip(np.ascontiguousarray(ne[l * walkers:(l + 1) * walkers])),
+                                      modes, 1.0, O.ip(out))
+        assert cc[l] == cnt
+        np.testing.assert_array_equal(ci[l, :cnt], out[:cnt])
+        assert (ci[l, cnt:] == -1).all()
+
+
+def test_gpu_apply_poses_matches_oracle(sw):
+    rng = np.random.default_rng(4)
+    pose_lig, xin, cen, rot, tors = [], [], [], [], []
+    for p in range(12):
+        l = p % 2
+        lig = sw["ligs"][l]
+        pose_lig.append(l)
+        xin.append(np.asarray(np.asarray(lig.xyz[0], np.float32), np.float64))
+        cen.append(np.asarray(rng.normal(size=3) * 5, np.float32).astype(np.float64))
+        rot.append(np.asarray(synth.quat_to_rot(synth.random_quaternion(rng)), np.float32)
+                   .astype(np.float64))
+        tors.append(np.asarray(rng.uniform(-math.pi, math.pi, lig.tree.n_torsions), np.float32)
+                    .astype(np.float64))
+    print("\n[apply] 12 poses", flush=True)
+    out, off = lib.apply_poses(sw["B"], pose_lig, xin, np.array(cen), np.array(rot), tors)
+    for p in range(12):
+        lig = sw["ligs"][pose_lig[p]]
+        ref = O.ligand_apply(lig.tree, xin[p], cen[p], rot[p].reshape(3, 3).ravel(), tors[p])
+        got = out[off[p]:off[p] + lig.n_atoms]
+        assert np.abs(got - ref).max() <= 1e-4, (p, np.abs(got - ref).max())

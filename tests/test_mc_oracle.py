@@ -1,0 +1,155 @@
+"""The oracle's Monte-Carlo walker (searcher.cpp:46-223) pinned by what the
+search must satisfy whatever the random stream:
+
+  * walker 0 is exactly one full L-BFGS of the input pose and its pscore
+    (searcher.cpp:105-123);
+  * randomize only moves the ligand rigidly and about its rotatable bonds
+    (every bonded distance kept), lands inside the box, and its vdW leaf is
+    below 100 (searcher.cpp:60-78);
+  * the best pose's pscore is the pscore of the best pose, never above the
+    walker's first optimised pose, and the pose lies in the box;
+  * with the optimiser off, each step costs exactly one evaluation and each
+    accepted improvement one more (searcher.cpp:159-172);
+  * at a vanishing temperature the walk is greedy: its current pscore never
+    rises, so the best is the last accepted state;
+  * the same seed gives the same walk, another seed another walk.
+
+The Philox stream itself is pinned by the Random123 known-answer vectors in
+test_oracle_golden.py.  Bit-level agreement with the reference's mt19937
+stream is not possible (std:: distributions are implementation-defined).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pxdock_amd import ffi, lib, scoring, synth
+
+
+@pytest.fixture(scope="module")
+def mcw():
+    rec = synth.make_receptor(3, n_heavy=500, radius=16.0, pocket_offset=7.0, n_ring6=3,
+                              n_ring5=1, n_leaves=3)
+    lig = synth.make_ligand(44, n_heavy=16, n_torsions=4, n_rings=1)
+    box = lib.decide_grid_dims(rec.pocket_center, [16.0] * 3, 0.5)
+    sp = scoring.load_pose_selection()
+    s = O.System(rec, lig, None)
+    # the input conformation placed at the pocket centre
+    x0 = synth.ligand_apply(lig.tree, lig.xyz[0], rec.pocket_center, np.eye(3),
+                            np.zeros(lig.tree.n_torsions))
+    return dict(rec=rec, lig=lig, box=box, sp=sp, s=s, x0=np.asarray(x0))
+
+
+def _no_opt(seed, steps, temperature=1.2):
+    prm = ffi.mc_params(seed=seed, max_nsteps=steps)
+    prm.relax = ffi.lbfgs_params(0)
+    prm.full = ffi.lbfgs_params(0)
+    prm.temperature = temperature
+    return prm
+
+
+def _in_box(x, box):
+    lo, hi = np.array(box.init_xyz), np.array(box.oppo_xyz)
+    return bool(((x > lo) & (x < hi)).all())
+
+
+def _bond_lengths(lig, x):
+    ij = np.asarray(lig.bond_ij).reshape(-1, 2)
+    return np.linalg.norm(x[ij[:, 0]] - x[ij[:, 1]], axis=1)
+
+
+def test_walker0_is_one_full_lbfgs(mcw):
+    prm = ffi.mc_params(seed=3, max_nsteps=50, full_iters=30)
+    out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], prm, mcw["box"], 0, 0, mcw["x0"])
+    ref = mcw["s"].lbfgs(mcw["x0"], None, prm.full)
+    assert out["steps"] == 0
+    assert out["nevals"] == ref["nevals"]
+    assert out["energy"] == ref["energy"]
+    np.testing.assert_array_equal(out["xyz"], ref["xyz"])
+    rx = O.receptor_apply(mcw["rec"].leaves, mcw["rec"].xyz, ref["tors"])
+    ps, _ = O.pscore(mcw["sp"], mcw["rec"], rx, mcw["lig"], ref["xyz"])
+    assert out["pscore"] == ps
+
+
+def test_randomize_is_rigid_in_box_and_clash_free(mcw):
+    lig = mcw["lig"]
+    b0 = _bond_lengths(lig, mcw["x0"])
+    seen = []
+    for w in range(1, 7):
+        out = O.mc_walker(mcw["s"], lig, mcw["sp"], _no_opt(11, 0), mcw["box"], w, 0, mcw["x0"])
+        x = out["xyz"]
+        assert out["steps"] == 0 and out["nevals"] == 1
+        np.testing.assert_allclose(_bond_lengths(lig, x), b0, rtol=0, atol=1e-9)
+        assert _in_box(x, mcw["box"])
+        vdw = O.lib().or_pscore_vdw(O.byref(mcw["sp"]), O.byref(mcw["rec"].to_c()),
+                                    O.dp(O.f64(mcw["rec"].xyz)), O.byref(lig.to_c()),
+                                    O.dp(O.f64(x)))
+        assert vdw < 100.0
+        seen.append(x)
+    # distinct walkers draw distinct poses
+    for a in range(len(seen)):
+        for b in range(a + 1, len(seen)):
+            assert np.abs(seen[a] - seen[b]).max() > 1e-3
+
+
+def test_walk_bookkeeping_without_optimiser(mcw):
+    steps = 25
+    for w in (1, 2, 3):
+        out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(5, steps), mcw["box"], w, 0,
+                          mcw["x0"])
+        first = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(5, 0), mcw["box"], w, 0,
+                            mcw["x0"])
+        assert out["steps"] == steps
+        # one evaluation for the start, one per step, one per refine
+        assert 1 + steps <= out["nevals"] <= 1 + 2 * steps
+        assert out["pscore"] <= first["pscore"]
+        assert _in_box(out["xyz"], mcw["box"])
+        rx = O.receptor_apply(mcw["rec"].leaves, mcw["rec"].xyz, out["tors"])
+        ps, _ = O.pscore(mcw["sp"], mcw["rec"], rx, mcw["lig"], out["xyz"])
+        assert out["pscore"] == pytest.approx(ps, rel=1e-12, abs=1e-12)
+        e, _, _ = mcw["s"].energy(out["xyz"], out["tors"])
+        assert out["energy"] == pytest.approx(e, rel=1e-12, abs=1e-12)
+        b0 = _bond_lengths(mcw["lig"], mcw["x0"])
+        np.testing.assert_allclose(_bond_lengths(mcw["lig"], out["xyz"]), b0, atol=1e-8)
+
+
+def test_greedy_walk_at_zero_temperature(mcw):
+    """T -> 0: Metropolis never accepts a worse pscore, and with the optimiser
+    off every accepted step that improves on the best is the new best, so the
+    number of evaluations equals 1 + steps + (number of improvements)."""
+    steps = 30
+    prm = _no_opt(9, steps, temperature=1e-300)
+    out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], prm, mcw["box"], 2, 0, mcw["x0"])
+    # replay the prefix walks: the best pscore is non-increasing in the number of steps
+    prev = math.inf
+    for k in range(0, steps + 1, 5):
+        o = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(9, k, 1e-300), mcw["box"], 2,
+                        0, mcw["x0"])
+        assert o["pscore"] <= prev
+        prev = o["pscore"]
+    assert prev == out["pscore"]
+
+
+def test_same_seed_same_walk_other_seed_other_walk(mcw):
+    a = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(21, 10), mcw["box"], 1, 0, mcw["x0"])
+    b = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(21, 10), mcw["box"], 1, 0, mcw["x0"])
+    c = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(22, 10), mcw["box"], 1, 0, mcw["x0"])
+    d = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(21, 10), mcw["box"], 1, 1, mcw["x0"])
+    np.testing.assert_array_equal(a["xyz"], b["xyz"])
+    assert a["pscore"] == b["pscore"]
+    assert np.abs(a["xyz"] - c["xyz"]).max() > 1e-3
+    assert np.abs(a["xyz"] - d["xyz"]).max() > 1e-3  # the ligand index is part of the counter
+
+
+def test_full_search_improves_on_the_start(mcw):
+    prm = ffi.mc_params(seed=1, max_nsteps=4, relax_iters=8, full_iters=20)
+    for w in (1, 2):
+        out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], prm, mcw["box"], w, 0, mcw["x0"])
+        p0 = ffi.mc_params(seed=1, max_nsteps=0, relax_iters=8, full_iters=20)
+        first = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], p0, mcw["box"], w, 0, mcw["x0"])
+        assert out["pscore"] <= first["pscore"]
+        assert out["nevals"] >= first["nevals"] + 4
+        rx = O.receptor_apply(mcw["rec"].leaves, mcw["rec"].xyz, out["tors"])
+        ps, _ = O.pscore(mcw["sp"], mcw["rec"], rx, mcw["lig"], out["xyz"])
+        assert out["pscore"] == pytest.approx(ps, rel=1e-12, abs=1e-12)
