@@ -227,10 +227,29 @@ void pxd_decide_grid_dims(const double center[3], const double size[3], double s
 typedef struct pxd_receptor pxd_receptor;
 
 /* Copies the receptor's coordinates, force field, interaction sets and leaves
- * to device memory.  xyz is [n_atoms][3]. */
+ * to device memory.  xyz is [n_atoms][3].
+ * Leaf energy tables: the receptor terms whose movable atoms all belong to
+ * one leaf depend on that leaf's angle only; their sum is tabulated per leaf
+ * in FP64 (value and derivative, cubic Hermite lookup) when the table
+ * reproduces direct FP64 evaluation at every interval midpoint within 1e-8
+ * (value) and 1e-6 (derivative), relative and floored at 1; otherwise these
+ * terms stay evaluated pair by pair (binding_system_interactions::
+ * put_gradients, interaction.cpp:271-298, gives the same energy either way). */
 pxd_status pxd_receptor_create(const pxd_mol* mol, const double* xyz,
                                const pxd_leaves* leaves, pxd_receptor** out);
 void pxd_receptor_destroy(pxd_receptor* r);
+
+/* Switches the leaf tables off (0) or back on (1) for later launches; the
+ * switch is not synchronised with launches already queued on the receptor.
+ * PXD_ERR_STATE when asked to enable tables that were not built.
+ * No counterpart in the reference: it selects between two evaluations of
+ * interaction.cpp:316-350 that agree within the bounds above. */
+pxd_status pxd_receptor_leaf_tables(pxd_receptor* r, int32_t enable);
+
+/* Leaves, table nodes per leaf (0: direct evaluation) and the worst midpoint
+ * errors measured when the tables were built. */
+pxd_status pxd_receptor_info(const pxd_receptor* r, int32_t* n_leaves, int32_t* table_nodes,
+                             double* table_err_e, double* table_err_d);
 
 /* ------------------------------------------------------------------------ */
 /* Grid maps (receptor_cache, grid.h:60-133)                                 */

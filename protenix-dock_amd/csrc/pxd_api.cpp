@@ -120,6 +120,80 @@ double h_pair(double qq, double sig, double e4, double d) {
   return qq / (d + 1e-20) * 332.0637141491396 + (u6 * u6 - u6) * e4;
 }
 
+// Leaf energy tables (pxd_leaf.hpp): nodes at 2*pi*i/n, n doubled from 2048
+// until cubic Hermite interpolation reproduces the directly evaluated FP64
+// value and derivative at every midpoint within kTabTolE / kTabTolD
+// (relative, floored at 1), else the receptor keeps direct evaluation.
+constexpr double kTabTolE = 1e-8, kTabTolD = 1e-6;
+constexpr int kTabMinNodes = 2048, kTabMaxNodes = 65536;
+
+pxd_status build_leaf_tables(pxd_receptor* r) {
+  const int L = r->n_leaves;
+  if (L == 0) return PXD_OK;
+  const double two_pi = 6.283185307179586;
+  double2* buf = nullptr;
+  const size_t cap = (size_t)L * kTabMaxNodes;
+  HIP_TRY(hipMalloc(&buf, 2 * cap * sizeof(double2)));
+  std::vector<double2> nodes, mids;
+  for (int n = kTabMinNodes; n <= kTabMaxNodes; n *= 2) {
+    double2* dn = buf;
+    double2* dm = buf + (size_t)L * n;
+    launch_leaf_table(r->dev, n, 0.0, dn, nullptr);
+    launch_leaf_table(r->dev, n, 0.5, dm, nullptr);
+    nodes.resize((size_t)L * n);
+    mids.resize((size_t)L * n);
+    hipError_t e1 = hipMemcpy(nodes.data(), dn, nodes.size() * sizeof(double2),
+                              hipMemcpyDeviceToHost);
+    hipError_t e2 = hipMemcpy(mids.data(), dm, mids.size() * sizeof(double2),
+                              hipMemcpyDeviceToHost);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+      (void)hipFree(buf);
+      return fail(PXD_ERR_HIP, "leaf tables: %s",
+                  hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+    }
+    const double h = two_pi / n;
+    double we = 0.0, wd = 0.0;
+    bool ok = true;
+    for (int k = 0; k < L && ok; ++k)
+      for (int i = 0; i < n; ++i) {
+        double2 a = nodes[(size_t)k * n + i], b = nodes[(size_t)k * n + (i + 1) % n];
+        double2 m = mids[(size_t)k * n + i];
+        double eh = 0.5 * (a.x + b.x) + 0.125 * h * (a.y - b.y);
+        double dh = 1.5 * (b.x - a.x) / h - 0.25 * (a.y + b.y);
+        double re = std::fabs(eh - m.x) / std::max(std::fabs(m.x), 1.0);
+        double rd = std::fabs(dh - m.y) / std::max(std::fabs(m.y), 1.0);
+        if (!(re <= kTabTolE) || !(rd <= kTabTolD)) {
+          ok = false;
+          we = re;
+          wd = rd;
+          break;
+        }
+        we = std::max(we, re);
+        wd = std::max(wd, rd);
+      }
+    r->tab_err_e = we;
+    r->tab_err_d = wd;
+    if (!ok) continue;
+    // keep the nodes in their own allocation
+    double2* tab = nullptr;
+    hipError_t he = hipMalloc(&tab, (size_t)L * n * sizeof(double2));
+    if (he == hipSuccess)
+      he = hipMemcpy(tab, nodes.data(), (size_t)L * n * sizeof(double2), hipMemcpyHostToDevice);
+    (void)hipFree(buf);
+    if (he != hipSuccess) {
+      if (tab) (void)hipFree(tab);
+      return fail(PXD_ERR_HIP, "leaf tables: %s", hipGetErrorString(he));
+    }
+    r->tab_mem = tab;
+    r->tab_n_built = n;
+    r->dev.tab = tab;
+    r->dev.tab_n = n;
+    return PXD_OK;
+  }
+  (void)hipFree(buf);
+  return PXD_OK;  // no table met the bound: direct evaluation
+}
+
 struct LbfgsWorkspace {
   std::mutex mu;
   float* hist = nullptr;
@@ -335,6 +409,76 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
     }
     rp_seg[s + 1] = (int)rp_pos.size();
   }
+  // order every term list by leaf class (one leaf k: class k; several: nl)
+  auto leaf_of_ref = [&](int rf) { return rf < 0 ? mov_leaf[-rf - 1] : -1; };
+  auto term_class = [&](const int* refs, int cnt) {
+    int c = -1;
+    for (int q = 0; q < cnt; ++q) {
+      int l = leaf_of_ref(refs[q]);
+      if (l < 0) continue;
+      if (c < 0) c = l;
+      else if (c != l) return nl;
+    }
+    return c < 0 ? nl : c;
+  };
+  auto by_class = [&](const std::vector<int>& cls, std::vector<int>& seg) {
+    std::vector<int> idx(cls.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cls[a] < cls[b]; });
+    seg.assign(nl + 2, 0);
+    for (int c : cls) seg[c + 1]++;
+    for (int k = 0; k <= nl; ++k) seg[k + 1] += seg[k];
+    seg.pop_back();  // [nl+1]: seg[nl] is the first cross term
+    return idx;
+  };
+  auto permute = [](auto& v, const std::vector<int>& idx, int width) {
+    auto old = v;
+    for (size_t i = 0; i < idx.size(); ++i)
+      for (int w = 0; w < width; ++w) v[i * width + w] = old[(size_t)idx[i] * width + w];
+  };
+  std::vector<int> bond_seg, angle_seg, dih_seg, mm_seg, leaf_slot(nl + 1, 0);
+  {
+    std::vector<int> cls(bab.size());
+    for (size_t i = 0; i < bab.size(); ++i) {
+      int r2[2] = {bab[i].x, bab[i].y};
+      cls[i] = term_class(r2, 2);
+    }
+    auto idx = by_class(cls, bond_seg);
+    permute(bab, idx, 1);
+    permute(bkr, idx, 1);
+  }
+  {
+    std::vector<int> cls(aabc.size());
+    for (size_t i = 0; i < aabc.size(); ++i) {
+      int r3[3] = {aabc[i].x, aabc[i].y, aabc[i].z};
+      cls[i] = term_class(r3, 3);
+    }
+    auto idx = by_class(cls, angle_seg);
+    permute(aabc, idx, 1);
+    permute(akt, idx, 1);
+  }
+  {
+    std::vector<int> cls(dabcd.size());
+    for (size_t i = 0; i < dabcd.size(); ++i) {
+      int r4[4] = {dabcd[i].x, dabcd[i].y, dabcd[i].z, dabcd[i].w};
+      cls[i] = term_class(r4, 4);
+    }
+    auto idx = by_class(cls, dih_seg);
+    permute(dabcd, idx, 1);
+    permute(dper, idx, 1);
+    permute(dkp, idx, 3);
+  }
+  {
+    std::vector<int> cls(mm.size());
+    for (size_t i = 0; i < mm.size(); ++i) {
+      int r2[2] = {mov_ref(mm[i].x), mov_ref(mm[i].y)};
+      cls[i] = term_class(r2, 2);
+    }
+    auto idx = by_class(cls, mm_seg);
+    permute(mm, idx, 1);
+  }
+  for (int sl = 0; sl < M; ++sl) leaf_slot[mov_leaf[sl] + 1] = sl + 1;
+  for (int k = 0; k < nl; ++k) leaf_slot[k + 1] = std::max(leaf_slot[k + 1], leaf_slot[k]);
   std::vector<float4> lorg(nl), lax(nl);
   for (int k = 0; k < nl; ++k) {
     lorg[k] = make_float4((float)lv->origin[3 * k], (float)lv->origin[3 * k + 1],
@@ -367,7 +511,9 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
          o_rppos = P.add(rp_pos), o_rplj = P.add(rp_lj), o_mm = P.add(mm),
          o_hydro = P.add(hydro), o_cat = P.add(cat), o_an = P.add(an), o_r5 = P.add(r5),
          o_r6 = P.add(r6), o_donc = P.add(donc), o_donn = P.add(donn), o_accc = P.add(accc),
-         o_accn = P.add(accn), o_slot = P.add(slot_of);
+         o_accn = P.add(accn), o_slot = P.add(slot_of), o_bseg = P.add(bond_seg),
+         o_aseg = P.add(angle_seg), o_dseg = P.add(dih_seg), o_mseg = P.add(mm_seg),
+         o_lslot = P.add(leaf_slot);
   pxd_receptor* r = new pxd_receptor();
   hipError_t he = hipMalloc(&r->mem, P.host.size());
   if (he != hipSuccess) {
@@ -431,10 +577,25 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   D.e_const = e_const;
   D.fixed_xyzq = dptr<float4>(B, o_fxyzq);
   D.fixed_lj = dptr<float2>(B, o_flj);
+  D.bond_seg = dptr<int>(B, o_bseg);
+  D.angle_seg = dptr<int>(B, o_aseg);
+  D.dih_seg = dptr<int>(B, o_dseg);
+  D.mm_seg = dptr<int>(B, o_mseg);
+  D.leaf_slot = dptr<int>(B, o_lslot);
+  D.tab_n = 0;
+  D.tab = nullptr;
+  r->tab_mem = nullptr;
+  r->tab_n_built = 0;
+  r->tab_err_e = r->tab_err_d = 0.0;
   r->n_atoms = n;
   r->n_leaves = nl;
   r->n_mov = M;
   r->h_xyz = nullptr;
+  pxd_status st = build_leaf_tables(r);
+  if (st != PXD_OK) {
+    pxd_receptor_destroy(r);
+    return st;
+  }
   *out = r;
   return PXD_OK;
 }
@@ -442,7 +603,27 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
 void pxd_receptor_destroy(pxd_receptor* r) {
   if (!r) return;
   (void)hipFree(r->mem);
+  if (r->tab_mem) (void)hipFree(r->tab_mem);
   delete r;
+}
+
+pxd_status pxd_receptor_leaf_tables(pxd_receptor* r, int32_t enable) {
+  if (!r) return fail(PXD_ERR_ARG, "leaf_tables: null receptor");
+  if (enable && !r->tab_mem && r->n_leaves > 0)
+    return fail(PXD_ERR_STATE, "leaf_tables: no table met the accuracy bound at build");
+  r->dev.tab = enable && r->tab_mem ? static_cast<const double2*>(r->tab_mem) : nullptr;
+  r->dev.tab_n = r->dev.tab ? r->tab_n_built : 0;
+  return PXD_OK;
+}
+
+pxd_status pxd_receptor_info(const pxd_receptor* r, int32_t* n_leaves, int32_t* table_nodes,
+                             double* table_err_e, double* table_err_d) {
+  if (!r) return fail(PXD_ERR_ARG, "receptor_info: null receptor");
+  if (n_leaves) *n_leaves = r->n_leaves;
+  if (table_nodes) *table_nodes = r->dev.tab ? r->tab_n_built : 0;
+  if (table_err_e) *table_err_e = r->tab_err_e;
+  if (table_err_d) *table_err_d = r->tab_err_d;
+  return PXD_OK;
 }
 
 // ------------------------------------------------------------------ grid

@@ -306,7 +306,12 @@ __device__ inline void receptor_torsion2xyz(const DevReceptor& R, PoseLds& s) {
 __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
   double e = 0.0;
   const int lane = lane_id();
-  for (int b = lane; b < R.n_bonds; b += kWave) {
+  // with leaf tables only the terms spanning two or more leaves remain
+  const bool tab = R.tab != nullptr;
+  const int L = R.n_leaves;
+  const int b0 = tab ? R.bond_seg[L] : 0, a0 = tab ? R.angle_seg[L] : 0,
+            t0 = tab ? R.dih_seg[L] : 0, m0 = tab ? R.mm_seg[L] : 0;
+  for (int b = b0 + lane; b < R.n_bonds; b += kWave) {
     int2 ab = R.bond_ab[b];
     float2 kr = R.bond_kr[b];
     float pa[3], pb[3];
@@ -320,7 +325,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, ab.x, d, -de);
     rec_grad_add(s, ab.y, d, de);
   }
-  for (int a = lane; a < R.n_angles; a += kWave) {
+  for (int a = a0 + lane; a < R.n_angles; a += kWave) {
     int4 id = R.angle_abc[a];
     float2 kt = R.angle_kt[a];
     float pi[3], pj[3], pk[3], fi[3], fk[3];
@@ -336,7 +341,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, id.z, fk, de);
     rec_grad_add(s, id.y, fj, -de);
   }
-  for (int t = lane; t < R.n_dih; t += kWave) {
+  for (int t = t0 + lane; t < R.n_dih; t += kWave) {
     int4 id = R.dih_abcd[t];
     float p[4][3], g[4][3];
     rec_pos(R, s, id.x, p[0]);
@@ -352,7 +357,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, id.w, g[3], -dedt);
   }
   // movable-vs-movable pairs
-  for (int k = lane; k < R.n_mm; k += kWave) {
+  for (int k = m0 + lane; k < R.n_mm; k += kWave) {
     int4 mm = R.mm[k];
     float cs = mm.z ? kScale14Coul : 1.f, vs = mm.z ? kScale14Vdw : 1.f;
     float4 qa = R.xyzq[R.mov_atom[mm.x]], qb = R.xyzq[R.mov_atom[mm.y]];
@@ -366,6 +371,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, mov_ref(mm.y), r, de);
   }
   __syncthreads();
+  if (tab) return e;  // movable-vs-fixed pairs are all inside the leaf tables
   // movable-vs-fixed pairs: per-slot segments padded to the wave width, the
   // slot's gradient stays in registers and is reduced once per segment
   for (int m = 0; m < R.n_mov; ++m) {
@@ -606,6 +612,33 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
   return e;
 }
 
+// Cubic Hermite lookup of leaf k's table at theta (any real theta).
+__device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, double* e,
+                                   double* d) {
+  const int n = R.tab_n;
+  const double two_pi = 6.283185307179586, h = two_pi / n;
+  double q = fmod((double)theta, two_pi);  // exact reduction
+  if (!(q == q)) {                         // NaN in, NaN out
+    *e = *d = q;
+    return;
+  }
+  if (q < 0.0) q += two_pi;
+  double u = q / h;
+  double fl = floor(u);
+  double t = u - fl;
+  int i0 = (int)fl;
+  if (i0 >= n) i0 -= n;
+  int i1 = i0 + 1 == n ? 0 : i0 + 1;
+  double2 a = R.tab[(long long)k * n + i0], b = R.tab[(long long)k * n + i1];
+  double t2 = t * t, t3 = t2 * t;
+  double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2,
+         h11 = t3 - t2;
+  *e = h00 * a.x + h10 * h * a.y + h01 * b.x + h11 * h * b.y;
+  double g00 = 6 * t2 - 6 * t, g10 = 3 * t2 - 4 * t + 1, g01 = -6 * t2 + 6 * t,
+         g11 = 3 * t2 - 2 * t;
+  *d = (g00 * a.x + g01 * b.x) / h + g10 * a.y + g11 * b.y;
+}
+
 // binding_system_interactions::put_gradients (interaction.cpp:271-298) for
 // the pose whose ligand coordinates and receptor torsions are in s.xyz /
 // s.tors.  Gradients are written to s.grad / s.torsg (overwrite).  Returns
@@ -622,8 +655,14 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
   e += ligand_intra(L, li, s);
   e += inter(R, G, li, s);
   __syncthreads();
-  // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf
-  for (int k = lane; k < R.n_leaves; k += kWave) s.torsg[k] = 0.f;
+  // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf;
+  // a leaf table contributes its energy and dT/dtheta directly
+  for (int k = lane; k < R.n_leaves; k += kWave) {
+    double te = 0.0, td = 0.0;
+    if (R.tab) leaf_lookup(R, k, s.tors[k], &te, &td);
+    e += te;
+    s.torsg[k] = (float)td;
+  }
   __syncthreads();
   for (int m = lane; m < R.n_mov; m += kWave) {
     float v = s.jac[3 * m] * s.movg[3 * m] + s.jac[3 * m + 1] * s.movg[3 * m + 1] +

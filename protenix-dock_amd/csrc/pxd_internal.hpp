@@ -74,6 +74,22 @@ struct DevReceptor {
   const float2* fixed_lj;
   // FP64 energy of receptor terms touching no movable atom (pose independent)
   double e_const;
+  // Term lists above are ordered by leaf class: the terms whose movable atoms
+  // all belong to leaf k occupy [seg[k], seg[k+1]); terms spanning two or more
+  // leaves follow from seg[n_leaves].  Movable slots of leaf k are
+  // [leaf_slot[k], leaf_slot[k+1]).
+  const int* bond_seg;        // [L+1]
+  const int* angle_seg;
+  const int* dih_seg;
+  const int* mm_seg;
+  const int* leaf_slot;
+  // Leaf energy tables: with every other atom fixed, the energy of the terms
+  // of leaf k is a 2*pi-periodic function T_k(theta_k).  tab[k*tab_n + i]
+  // holds (T_k, dT_k/dtheta) at theta = 2*pi*i/tab_n in FP64; a lookup is a
+  // cubic Hermite interpolation.  tab == nullptr: every term is evaluated
+  // directly.
+  int tab_n;
+  const double2* tab;
 };
 
 // Per-ligand record of the packed batch.
@@ -145,6 +161,9 @@ struct pxd_receptor {
   int n_atoms;
   void* mem;                  // single device allocation backing dev arrays
   size_t mem_bytes;
+  void* tab_mem;              // leaf energy tables (nullptr: none)
+  int tab_n_built;            // nodes per leaf of the built tables (0: none)
+  double tab_err_e, tab_err_d; // worst midpoint errors measured at build
   // host copies needed by other entry points
   float* h_xyz;               // initial coordinates
   int n_leaves;

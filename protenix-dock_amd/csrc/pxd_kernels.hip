@@ -6,6 +6,7 @@
 #include "pxd_kernels.hpp"
 #include "pxd_pscore.hpp"
 #include "pxd_mc.hpp"
+#include "pxd_leaf.hpp"
 
 namespace pxd {
 
@@ -1012,6 +1013,25 @@ __global__ __launch_bounds__(64) void k_apply(DevLigands L, ApplyArgs A) {
 void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st) {
   if (a.n_poses <= 0) return;
   hipLaunchKernelGGL(k_apply, dim3(a.n_poses), dim3(kWave), 0, st, L, a);
+}
+
+// ============================================================== receptor leaf tables
+// One wave per (leaf, sample): T_k and dT_k/dtheta at theta = 2*pi*(i + shift)/n
+// (shift 0: table nodes, 0.5: midpoints used to measure the interpolation error).
+
+__global__ __launch_bounds__(64) void k_leaf_table(DevReceptor R, int n, double shift,
+                                                   double2* out) {
+  const int k = blockIdx.y, i = blockIdx.x;
+  if (k >= R.n_leaves || i >= n) return;
+  double e, d;
+  leaf_energy(R, k, 6.283185307179586 * ((double)i + shift) / n, &e, &d);
+  if (lane_id() == 0) out[(long long)k * n + i] = make_double2(e, d);
+}
+
+void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
+                       hipStream_t st) {
+  if (R.n_leaves <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_leaf_table, dim3(n, R.n_leaves), dim3(kWave), 0, st, R, n, shift, out);
 }
 
 }  // namespace pxd

@@ -137,5 +137,9 @@ struct ApplyArgs {
   float* xyz_out;
 };
 void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st);
+// Receptor leaf energy tables: out[k*n + i] = (T_k, dT_k/dtheta) at
+// theta = 2*pi*(i + shift)/n.
+void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
+                       hipStream_t st);
 
 }  // namespace pxd

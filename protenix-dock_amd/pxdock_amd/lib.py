@@ -62,6 +62,8 @@ def load():
         "pxd_lbfgs_batch": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_pscore_batch": (i, [P, P, P, i, P, P, P, P, P, P, P]),
         "pxd_lbfgs_plain": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_receptor_leaf_tables": (i, [P, i]),
+        "pxd_receptor_info": (i, [P, i32p, i32p, f64p, f64p]),
         "pxd_mc_default_params": (None, [P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
@@ -121,6 +123,17 @@ class Receptor:
                                          C.byref(h)))
         self.h = h
         self.n_leaves = mol.leaves.n_leaves if mol.leaves is not None else 0
+
+    def table_info(self):
+        """(n_leaves, table nodes per leaf or 0, worst midpoint errors e, d)."""
+        nl, nn = C.c_int32(), C.c_int32()
+        ee, ed = C.c_double(), C.c_double()
+        check(load().pxd_receptor_info(self.h, C.byref(nl), C.byref(nn), C.byref(ee),
+                                       C.byref(ed)))
+        return nl.value, nn.value, ee.value, ed.value
+
+    def set_leaf_tables(self, enable):
+        check(load().pxd_receptor_leaf_tables(self.h, 1 if enable else 0))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

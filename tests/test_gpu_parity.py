@@ -255,3 +255,36 @@ def test_gpu_pscore_matches_oracle(world):
         rs, rt = O.pscore(sp, world["rec"], rx, world["ligs"][pose_lig[p]], xyz[p])
         np.testing.assert_allclose(t[p], rt, rtol=1e-4, atol=1e-4)
         assert abs(s[p] - rs) <= 1e-4 * max(abs(rs), 1.0)
+
+
+# ---------------------------------------------------------------- receptor leaf tables
+
+def test_gpu_leaf_tables_match_direct_and_oracle(world):
+    """Leaf energy tables (FP64 Hermite) vs direct pair-by-pair evaluation and
+    vs the FP64 oracle on the same poses, including torsions far outside
+    [-pi, pi] (the tables are periodic)."""
+    nl, nodes, ee, ed = world["R"].table_info()
+    print(f"\nleaf tables: {nl} leaves x {nodes} nodes, midpoint errors {ee:.2e} / {ed:.2e}")
+    assert nl == 8 and nodes >= 2048 and ee <= 1e-8 and ed <= 1e-6
+    pose_lig, xyz, tors = _poses(world, 6, 41, spread=2.0)
+    tors = _f32(tors * 5.0)
+    B = lib.LigandBatch(world["ligs"])
+    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    e1, g1, t1 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+    world["R"].set_leaf_tables(False)
+    try:
+        e0, g0, t0 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+    finally:
+        world["R"].set_leaf_tables(True)
+    og = _oracle_grid(world)
+    systems = [O.System(world["rec"], lig, og) for lig in world["ligs"]]
+    for p in range(P.n_poses):
+        re, rg, rt = systems[pose_lig[p]].energy(xyz[p], tors[p])
+        assert _close_e(e1[p], re), (p, e1[p], e0[p], re)
+        assert abs(e1[p] - e0[p]) <= 1e-5 * max(abs(e0[p]), 1.0), (p, e1[p], e0[p])
+        o = P.pose_atom_off[p]
+        n = world["ligs"][pose_lig[p]].n_atoms
+        full1 = np.concatenate([g1.reshape(-1, 3)[o:o + n].ravel(), t1[p]])
+        fullr = np.concatenate([rg.ravel(), rt])
+        assert _grad_ok(full1, fullr), (p, np.abs(full1 - fullr).max())
+        np.testing.assert_allclose(t1[p], t0[p], rtol=1e-4, atol=1e-3)

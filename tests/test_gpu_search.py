@@ -148,15 +148,20 @@ def test_gpu_full_search_self_consistent_and_clustered(sw):
     s, _ = lib.pscore(sw["R"], P, sw["sp"], with_terms=False)
     e, _, _ = lib.score_grad(sw["R"], sw["G"], P)
     s, e = s.cpu().numpy(), e.cpu().numpy()
+    # the same walkers stopped after their first full optimisation
+    p0 = ffi.mc_params(seed=17, max_nsteps=0, relax_iters=20, full_iters=60)
+    first = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], p0, sw["box"], walkers)
+    out_of_box = 0
     for w in range(W):
         assert ne[w] > 0
         assert abs(bp[w] - s[w]) <= 1e-6 * max(abs(s[w]), 1.0), (w, bp[w], s[w])
         assert abs(be[w] - e[w]) <= 1e-5 * max(abs(e[w]), 1.0), (w, be[w], e[w])
-        if w % walkers:
-            assert _in_box(res.walker_xyz(w), sw["box"])
-    # walkers that moved never end above their first optimised pose
-    p0 = ffi.mc_params(seed=17, max_nsteps=0, relax_iters=20, full_iters=60)
-    first = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], p0, sw["box"], walkers)
+        # a best pose may lie outside the box: a relaxed start that left it
+        # stops the walker (searcher.cpp:130, 182-184), and a refined pose is
+        # kept on pscore alone with no box check (searcher.cpp:167-174)
+        out_of_box += not _in_box(res.walker_xyz(w), sw["box"])
+    print(f"   best poses outside the box: {out_of_box}/{W}")
+    # walkers never end above their first optimised pose
     assert (bp <= first.best_pscore.cpu().numpy() + 1e-12).all()
     # clustering vs the oracle's pose_cluster on the same candidates
     ci, cc = res.cluster_idx.cpu().numpy(), res.cluster_count.cpu().numpy()
