@@ -790,6 +790,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   std::vector<float4> dkp, tscoef;
   std::vector<int> frag, torsf, perm, sets, rings, tsmoff;
   std::vector<int2> dons;
+  std::vector<uint4> pmask;
   std::vector<int> natoms(nlig), ntors(nlig);
   uint8_t used[PXD_NUM_VDW_TYPES] = {0};
   for (int l = 0; l < nlig; ++l) {
@@ -856,6 +857,33 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
       pairs.push_back(uint32_t(m.pairs14[2 * p]) | uint32_t(m.pairs14[2 * p + 1]) << 8 | 1u << 16);
     for (int p = 0; p < m.n_pairs; ++p)
       pairs.push_back(uint32_t(m.pairs[2 * p]) | uint32_t(m.pairs[2 * p + 1]) << 8);
+    // partner masks: each listed pair marked from both ends; a list that names
+    // a pair twice (or an atom with itself) keeps the list form
+    {
+      std::vector<uint32_t> mk((size_t)n * 8, 0u);
+      bool mask_ok = true;
+      for (int pass = 0; pass < 2 && mask_ok; ++pass) {
+        int cnt = pass == 0 ? m.n_pairs : m.n_pairs14;
+        const int32_t* ij = pass == 0 ? m.pairs : m.pairs14;
+        for (int p = 0; p < cnt; ++p) {
+          int i = ij[2 * p], j = ij[2 * p + 1];
+          uint32_t* wi = &mk[(size_t)i * 8 + 4 * pass];
+          uint32_t* wj = &mk[(size_t)j * 8 + 4 * pass];
+          if (i == j || (wi[j >> 5] >> (j & 31) & 1u)) {
+            mask_ok = false;
+            break;
+          }
+          wi[j >> 5] |= 1u << (j & 31);
+          wj[i >> 5] |= 1u << (i & 31);
+        }
+      }
+      I.pmode = mask_ok ? 1 : 0;
+      for (int a = 0; a < n; ++a)
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t* w = &mk[(size_t)a * 8 + 4 * h];
+          pmask.push_back(mask_ok ? make_uint4(w[0], w[1], w[2], w[3]) : make_uint4(0, 0, 0, 0));
+        }
+    }
     // fragment tree
     I.frag_off = (int)torsf.size();
     if (trees) {
@@ -931,7 +959,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
          o_dijkl = P.add(dijkl), o_dper = P.add(dper), o_dkp = P.add(dkp), o_pairs = P.add(pairs),
          o_frag = P.add(frag), o_torsf = P.add(torsf), o_perm = P.add(perm), o_sets = P.add(sets),
          o_rings = P.add(rings), o_dons = P.add(dons), o_tsc = P.add(tscoef),
-         o_tsm = P.add(tsmoff), o_tsi = P.add(tsijkl);
+         o_tsm = P.add(tsmoff), o_tsi = P.add(tsijkl), o_pmask = P.add(pmask);
   pxd_ligand_batch* b = new pxd_ligand_batch();
   hipError_t he = hipMalloc(&b->mem, P.host.size());
   if (he == hipSuccess) he = hipMemcpy(b->mem, P.host.data(), P.host.size(), hipMemcpyHostToDevice);
@@ -964,6 +992,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   D.ts_coef = dptr<float4>(B, o_tsc);
   D.ts_moff = dptr<int>(B, o_tsm);
   D.ts_ijkl = dptr<uint32_t>(B, o_tsi);
+  D.pmask = dptr<uint4>(B, o_pmask);
   b->mem_bytes = P.host.size();
   b->n = nlig;
   b->h_natoms = new int[nlig];

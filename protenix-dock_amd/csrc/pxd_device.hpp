@@ -196,10 +196,12 @@ struct PoseLds {
   float* jac;    // [kMaxMovable*3]
   float* tors;   // [kMaxLeaves]
   float* torsg;  // [kMaxLeaves]
+  float4* movp;  // [kMaxMovable] charge, sigma, sqrt(epsilon) of each movable slot
 };
 
-#define PXD_POSE_LDS_FLOATS \
-  (pxd::kMaxAtoms * 3 * 2 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 9 + pxd::kMaxLeaves * 2)
+#define PXD_POSE_LDS_FLOATS                                                             \
+  (pxd::kMaxAtoms * 3 * 2 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 + \
+   pxd::kMaxLeaves * 2)
 
 __device__ inline PoseLds carve_pose_lds(float* base) {
   PoseLds s;
@@ -213,6 +215,7 @@ __device__ inline PoseLds carve_pose_lds(float* base) {
   s.jac = s.movg + kMaxMovable * 3;
   s.tors = s.jac + kMaxMovable * 3;
   s.torsg = s.tors + kMaxLeaves;
+  s.movp = reinterpret_cast<float4*>(s.torsg + kMaxLeaves);
   return s;
 }
 
@@ -297,6 +300,8 @@ __device__ inline void receptor_torsion2xyz(const DevReceptor& R, PoseLds& s) {
     s.jac[3 * m + 1] = j[1];
     s.jac[3 * m + 2] = j[2];
     s.movg[3 * m] = s.movg[3 * m + 1] = s.movg[3 * m + 2] = 0.f;
+    float2 l = R.lj[R.mov_atom[m]];
+    s.movp[m] = make_float4(p0.w, l.x, l.y, 0.f);
   }
 }
 
@@ -403,13 +408,41 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
   return e;
 }
 
-// intra_molecular_interactions::put_gradients on the ligand (interaction.cpp:170-196)
+// intra_molecular_interactions::put_gradients on the ligand (interaction.cpp:170-196):
+// bonds, angles and dihedrals by term lists (the first descriptors of every
+// list are loaded before any of them is used, later ones one iteration
+// ahead), non-bonded pairs by the list only for ligands without partner masks.
 __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, PoseLds& s) {
   double e = 0.0;
   const int lane = lane_id();
-  for (int b = lane; b < li.n_bonds; b += kWave) {
-    uint32_t ij = L.bond_ij[li.bond_off + b];
-    float2 kr = L.bond_kr[li.bond_off + b];
+  const uint32_t* bij = L.bond_ij + li.bond_off;
+  const float2* bkr = L.bond_kr + li.bond_off;
+  const uint32_t* aijk = L.angle_ijk + li.angle_off;
+  const float2* akt = L.angle_kt + li.angle_off;
+  const uint32_t* dq = L.dih_ijkl + li.dih_off;
+  const uint32_t* dp = L.dih_per + li.dih_off;
+  const float4* dkp = L.dih_kp + 3 * li.dih_off;
+  int b = lane, a = lane, t = lane;
+  uint32_t b_w = b < li.n_bonds ? bij[b] : 0u;
+  float2 b_k = b < li.n_bonds ? bkr[b] : make_float2(0.f, 0.f);
+  uint32_t a_w = a < li.n_angles ? aijk[a] : 0u;
+  float2 a_k = a < li.n_angles ? akt[a] : make_float2(0.f, 0.f);
+  uint32_t d_w = 0u, d_p = 0u;
+  float4 d_k[3];
+  if (t < li.n_dih) {
+    d_w = dq[t];
+    d_p = dp[t];
+    d_k[0] = dkp[3 * t];
+    d_k[1] = dkp[3 * t + 1];
+    d_k[2] = dkp[3 * t + 2];
+  }
+  for (; b < li.n_bonds; b += kWave) {
+    uint32_t ij = b_w;
+    float2 kr = b_k;
+    if (b + kWave < li.n_bonds) {
+      b_w = bij[b + kWave];
+      b_k = bkr[b + kWave];
+    }
     int i = ij & 255, j = (ij >> 8) & 255;
     float d[3] = {s.xyz[3 * j] - s.xyz[3 * i], s.xyz[3 * j + 1] - s.xyz[3 * i + 1],
                   s.xyz[3 * j + 2] - s.xyz[3 * i + 2]};
@@ -420,9 +453,13 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
     lig_grad_add(s, i, d, -de);
     lig_grad_add(s, j, d, de);
   }
-  for (int a = lane; a < li.n_angles; a += kWave) {
-    uint32_t ijk = L.angle_ijk[li.angle_off + a];
-    float2 kt = L.angle_kt[li.angle_off + a];
+  for (; a < li.n_angles; a += kWave) {
+    uint32_t ijk = a_w;
+    float2 kt = a_k;
+    if (a + kWave < li.n_angles) {
+      a_w = aijk[a + kWave];
+      a_k = akt[a + kWave];
+    }
     int i = ijk & 255, j = (ijk >> 8) & 255, k = (ijk >> 16) & 255;
     float fi[3], fk[3];
     float th = angle_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, fi, fk);
@@ -434,19 +471,28 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
     lig_grad_add(s, k, fk, de);
     lig_grad_add(s, j, fj, -de);
   }
-  for (int t = lane; t < li.n_dih; t += kWave) {
-    uint32_t q = L.dih_ijkl[li.dih_off + t];
+  for (; t < li.n_dih; t += kWave) {
+    uint32_t q = d_w, per = d_p;
+    float4 kp[3] = {d_k[0], d_k[1], d_k[2]};
+    if (t + kWave < li.n_dih) {
+      d_w = dq[t + kWave];
+      d_p = dp[t + kWave];
+      d_k[0] = dkp[3 * (t + kWave)];
+      d_k[1] = dkp[3 * (t + kWave) + 1];
+      d_k[2] = dkp[3 * (t + kWave) + 2];
+    }
     int i = q & 255, j = (q >> 8) & 255, k = (q >> 16) & 255, l = (q >> 24) & 255;
     float g[4][3];
     float th = dihedral_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, s.xyz + 3 * l, g[0],
                              g[1], g[2], g[3]);
     float dedt;
-    e += fourier(L.dih_per[li.dih_off + t], L.dih_kp + 3 * (li.dih_off + t), th, &dedt);
+    e += fourier(per, kp, th, &dedt);
     lig_grad_add(s, i, g[0], -dedt);
     lig_grad_add(s, j, g[1], -dedt);
     lig_grad_add(s, k, g[2], -dedt);
     lig_grad_add(s, l, g[3], -dedt);
   }
+  if (li.pmode) return e;  // pairs by partner masks (ligand_pairs_masked)
   for (int p = lane; p < li.n_pairs; p += kWave) {
     uint32_t w = L.pairs[li.pair_off + p];
     int i = w & 255, j = (w >> 8) & 255;
@@ -460,6 +506,80 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
                      la.y * lb.y * 4.f * vs, &de);
     lig_grad_add(s, i, r, -de);
     lig_grad_add(s, j, r, de);
+  }
+  return e;
+}
+
+// Ligand non-bonded pairs (interaction.cpp:127-168) by partner masks: lane
+// owns atoms lane and lane + 64 and visits every partner b in order (LDS
+// broadcast reads, no atomics); a pair listed as all-pairs and/or 1-4
+// contributes with the sum of its scales.  Each pair is met from both ends,
+// each end adding half its energy and its own gradient (g0 / g1, registers).
+__device__ inline double ligand_pairs_masked(const DevLigands& L, const LigInfo& li,
+                                             const PoseLds& s, float* g0, float* g1) {
+  const int lane = lane_id(), n = li.n_atoms;
+  const bool two = n > kWave;
+  const int a0 = lane, a1 = lane + kWave;
+  uint4 m0a = make_uint4(0, 0, 0, 0), m0f = m0a, m1a = m0a, m1f = m0a;
+  float x0[3] = {0.f, 0.f, 0.f}, x1[3] = {0.f, 0.f, 0.f}, q0 = 0.f, q1 = 0.f;
+  float2 l0 = make_float2(0.f, 0.f), l1 = l0;
+  const uint4* pm = L.pmask + 2ll * li.atom_off;
+  if (a0 < n) {
+    m0a = pm[2 * a0];
+    m0f = pm[2 * a0 + 1];
+    x0[0] = s.xyz[3 * a0];
+    x0[1] = s.xyz[3 * a0 + 1];
+    x0[2] = s.xyz[3 * a0 + 2];
+    q0 = s.q[a0];
+    l0 = s.lj[a0];
+  }
+  if (a1 < n) {
+    m1a = pm[2 * a1];
+    m1f = pm[2 * a1 + 1];
+    x1[0] = s.xyz[3 * a1];
+    x1[1] = s.xyz[3 * a1 + 1];
+    x1[2] = s.xyz[3 * a1 + 2];
+    q1 = s.q[a1];
+    l1 = s.lj[a1];
+  }
+  double e = 0.0;
+  for (int b = 0; b < n; ++b) {
+    const float xb = s.xyz[3 * b], yb = s.xyz[3 * b + 1], zb = s.xyz[3 * b + 2], qb = s.q[b];
+    const float2 lb = s.lj[b];
+    const int w = b >> 5;
+    const uint32_t bit = 1u << (b & 31);
+    {
+      uint32_t wa = w == 0 ? m0a.x : w == 1 ? m0a.y : w == 2 ? m0a.z : m0a.w;
+      uint32_t wf = w == 0 ? m0f.x : w == 1 ? m0f.y : w == 2 ? m0f.z : m0f.w;
+      bool ia = wa & bit, i4 = wf & bit;
+      if (ia || i4) {
+        float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+        float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+        float rx = xb - x0[0], ry = yb - x0[1], rz = zb - x0[2], de;
+        float ep = pair_energy(rx, ry, rz, q0 * qb * cs, (l0.x + lb.x) * 0.5f,
+                               l0.y * lb.y * 4.f * vs, &de);
+        e += 0.5 * (double)ep;
+        g0[0] -= rx * de;
+        g0[1] -= ry * de;
+        g0[2] -= rz * de;
+      }
+    }
+    if (two) {
+      uint32_t wa = w == 0 ? m1a.x : w == 1 ? m1a.y : w == 2 ? m1a.z : m1a.w;
+      uint32_t wf = w == 0 ? m1f.x : w == 1 ? m1f.y : w == 2 ? m1f.z : m1f.w;
+      bool ia = wa & bit, i4 = wf & bit;
+      if (ia || i4) {
+        float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+        float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+        float rx = xb - x1[0], ry = yb - x1[1], rz = zb - x1[2], de;
+        float ep = pair_energy(rx, ry, rz, q1 * qb * cs, (l1.x + lb.x) * 0.5f,
+                               l1.y * lb.y * 4.f * vs, &de);
+        e += 0.5 * (double)ep;
+        g1[0] -= rx * de;
+        g1[1] -= ry * de;
+        g1[2] -= rz * de;
+      }
+    }
   }
   return e;
 }
@@ -547,12 +667,11 @@ __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float
 // With a grid: map lookup + pairs against movable receptor atoms; without:
 // pairs against every receptor atom (fixed atoms by broadcast loads).
 __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const LigInfo& li,
-                               PoseLds& s) {
+                               PoseLds& s, float* g0, float* g1) {
   double e = 0.0;
   const int lane = lane_id();
   const int n = li.n_atoms;
-  // lane owns atoms lane and lane + 64
-  float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
+  // lane owns atoms lane and lane + 64; g0 / g1 carry their gradients
   const int a0 = lane, a1 = lane + kWave;
   if (G) {
     if (a0 < n) grid_get(*G, s.slot[a0], s.q[a0], s.xyz + 3 * a0, &e, g0);
@@ -578,9 +697,9 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
   }
   // movable receptor atoms: explicit pairs, their gradient reduced per slot
   for (int m = 0; m < R.n_mov; ++m) {
-    int atom = R.mov_atom[m];
-    float qm = R.xyzq[atom].w;
-    float2 lm = R.lj[atom];
+    const float4 mp = s.movp[m];
+    const float qm = mp.x;
+    const float2 lm = make_float2(mp.y, mp.z);
     float px = s.mov[3 * m], py = s.mov[3 * m + 1], pz = s.mov[3 * m + 2];
     float gx = 0.f, gy = 0.f, gz = 0.f;
     for (int h = 0; h < 2; ++h) {
@@ -653,7 +772,9 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
   if (lane == 0) e += R.e_const;
   __syncthreads();
   e += ligand_intra(L, li, s);
-  e += inter(R, G, li, s);
+  float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
+  if (li.pmode) e += ligand_pairs_masked(L, li, s, g0, g1);
+  e += inter(R, G, li, s, g0, g1);
   __syncthreads();
   // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf;
   // a leaf table contributes its energy and dT/dtheta directly

@@ -112,6 +112,7 @@ struct LigInfo {
   int ts_off, n_ts;
   int n_rot;
   int tsm_off;   // first entry of this ligand in ts_moff (n_ts + 1 entries)
+  int pmode;     // 1: non-bonded pairs as per-atom partner masks (pmask), 0: pair list
 };
 
 struct DevLigands {
@@ -139,6 +140,9 @@ struct DevLigands {
   const float4* ts_coef;      // torsion-strain coefs
   const int* ts_moff;         // per ligand n_ts+1 absolute offsets into ts_ijkl
   const uint32_t* ts_ijkl;    // packed atoms
+  // non-bonded partner masks, [atom_off + a][2]: bit b of word b/32 set when
+  // (a, b) is an all-pairs entry (.x..w of [0]) or a 1-4 entry ([1])
+  const uint4* pmask;
 };
 
 struct DevGrid {

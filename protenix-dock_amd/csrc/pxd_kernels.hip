@@ -123,7 +123,7 @@ __global__ __launch_bounds__(64) void k_score(DevReceptor R, DevLigands L,
                                               double* __restrict__ energy,
                                               float* __restrict__ lig_grad,
                                               float* __restrict__ tors_grad) {
-  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   PoseLds s = carve_pose_lds(lds);
   const int p = xcd_block(blockIdx.x, gridDim.x);
   if (p >= n_poses) return;
@@ -542,7 +542,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 template <int VPL>
 __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
                                               const DevGrid* __restrict__ G, LbfgsArgs A) {
-  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
   __shared__ double s_al[kMaxHistory];
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
                                                const float* __restrict__ tors,
                                                double* __restrict__ score,
                                                double* __restrict__ terms) {
-  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ double s_terms[PXD_NUM_TERMS];
   PoseLds s = carve_pose_lds(lds);
@@ -658,7 +658,7 @@ enum { PH_INIT = 0, PH_MUTATE = 1, PH_REFINE = 2 };
 template <int VPL>
 __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
                                            const DevGrid* __restrict__ G, McArgs A) {
-  __shared__ float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
   __shared__ double s_al[kMaxHistory];
