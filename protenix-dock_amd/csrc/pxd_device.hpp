@@ -170,15 +170,20 @@ __device__ inline float dihedral(const float* pi, const float* pj, const float* 
 __device__ inline float pair_energy(float rx, float ry, float rz, float qq, float sig,
                                     float e4, float* de_dr_r) {
   float d2 = rx * rx + ry * ry + rz * rz;
-  float d = sqrtf(d2);
-  float ec = safe_div(qq, d) * kCoulomb;
-  float dc = -safe_div(qq, d2 * d) * kCoulomb;
-  float u = safe_div(sig, d);
+  // 1/(d + eps) from one hardware reciprocal square root (1 ulp); at d = 0
+  // it saturates at 1/eps = 1e20 as safe_divide does (math.h:176-178).
+  // 1/(d^3 + eps) and 1/(d^2 + eps) become powers of it: equal to FP32
+  // rounding for every d > 1e-6 A.
+  float rd = fminf(__builtin_amdgcn_rsqf(d2), 1e20f);
+  float rd2 = rd * rd;
+  float qk = qq * kCoulomb;
+  float ec = qk * rd;
+  float dc = -qk * rd2 * rd;
+  float u = sig * rd;
   float u2 = u * u;
   float u6 = u2 * u2 * u2;
   float u12 = u6 * u6;
-  float irsqr = safe_div(1.f, d2);
-  float dv = (-12.f * u12 + 6.f * u6) * irsqr * e4;
+  float dv = (-12.f * u12 + 6.f * u6) * rd2 * e4;
   *de_dr_r = dc + dv;
   return ec + (u12 - u6) * e4;
 }
