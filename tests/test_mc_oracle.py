@@ -15,7 +15,7 @@ search must satisfy whatever the random stream:
   * the same seed gives the same walk, another seed another walk.
 
 The Philox stream itself is pinned by the Random123 known-answer vectors in
-test_oracle_golden.py.  Bit-level agreement with the reference's mt19937
+test_oracle_pins.py.  Bit-level agreement with the reference's mt19937
 stream is not possible (std:: distributions are implementation-defined).
 """
 import math
