@@ -67,16 +67,16 @@ def workload(n_lig, per_lig, workers):
     return rec, box, ligs, poses
 
 
-def flops_bytes_per_pose(lig, rec_pairs, n_mov, n_rec_bonded):
-    """Algorithmic work of one score+grad evaluation (counting: add/mul = 1,
-    FMA = 2, div/sqrt/transcendental = 1): grid gather 256 B per atom (2 maps x
-    8 corners x 4 FP32 channels); FLOPs as 160 per atom (lookup), 35 per
-    non-bonded pair, 25 per bond, 60 per angle, 150 per dihedral."""
+def flops_bytes_per_pose(lig, n_excl):
+    """Algorithmic work of one score+grad evaluation of a pose with N atoms
+    (counting add/mul = 1, FMA = 2, div/sqrt/transcendental = 1): bytes
+    256 N (2 maps x 8 corners x 4 FP32 channels per atom lookup); FLOPs
+    160 N + 35 P_nb + 25 N_bond + 60 N_angle + 150 N_dihedral + 60 N N_excl,
+    N_excl the movable receptor atoms paired with every ligand atom."""
     n = lig.n_atoms
     b = 256.0 * n
-    f = (160.0 * n + 35.0 * (len(lig.pairs) + len(lig.pairs14) + rec_pairs + n * n_mov)
-         + 25.0 * len(lig.bond_ij) + 60.0 * len(lig.angle_ijk) + 150.0 * len(lig.dih_ijkl)
-         + 100.0 * n_rec_bonded)
+    f = (160.0 * n + 35.0 * (len(lig.pairs) + len(lig.pairs14)) + 25.0 * len(lig.bond_ij)
+         + 60.0 * len(lig.angle_ijk) + 150.0 * len(lig.dih_ijkl) + 60.0 * n * n_excl)
     return f, b
 
 
@@ -94,8 +94,8 @@ def main():
     ap.add_argument("--no-lbfgs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every box here has a single MI355X; "
-              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU only: sharding the path across GPUs (by ligand) "
+              "is not built", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -140,12 +140,10 @@ def main():
     log(f"score+grad: {ms:.3f} ms/step -> {value:.4e} evals/s (finite {finite:.4f})")
 
     # ---------------- roofline of the score kernel
-    rec_pairs = len(rec.pairs) + len(rec.pairs14)
     n_mov = len(rec.leaves.atoms)
-    nb = len(rec.bond_ij) + len(rec.angle_ijk) + len(rec.dih_ijkl)
     F = Bt = 0.0
     for l, lig in enumerate(ligs):
-        f, b = flops_bytes_per_pose(lig, rec_pairs, n_mov, nb)
+        f, b = flops_bytes_per_pose(lig, n_mov)
         F += f * args.poses_per_ligand
         Bt += b * args.poses_per_ligand
     t_s = ms / 1e3

@@ -305,8 +305,8 @@ pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* grid,
 /* Batched persistent L-BFGS (lbfgs_step::apply, optimizer.cpp:429-560):
  * xyz and tors are optimised IN PLACE.  Outputs per pose: energy (final
  * evaluation), nevals, converged (1/0).  Many uneven instances run in one
- * launch from a device work queue.  workspace may be NULL (the library then
- * allocates history storage for its slots). */
+ * launch from a device work queue.  The library keeps the history storage of
+ * its work slots between calls (grown on demand, one per process). */
 pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* grid,
                            const pxd_ligand_batch* b, const pxd_lbfgs_params* prm,
                            int32_t n_poses, const int32_t* pose_lig,
