@@ -80,6 +80,86 @@ def flops_bytes_per_pose(lig, n_excl):
     return f, b
 
 
+def _make_search_ligand(i):
+    from pxdock_amd import synth
+    rng = np.random.default_rng(30000 + i + 7919)
+    nh = int(rng.integers(10, 61))
+    nt = int(rng.integers(0, 16))
+    nr = int(rng.integers(0, 4))
+    return synth.make_ligand(30000 + i, n_heavy=nh, n_torsions=nt, n_rings=nr)
+
+
+def search_params(seed, steps):
+    """Full-search chain parameters: Gaussian mutation (translation sigma 1 A,
+    rotation-vector sigma 0.3 rad, one torsion uniform), local L-BFGS with
+    m = 8, at most 300 iterations, |g|inf <= 1e-3, Metropolis T = 1.2."""
+    from pxdock_amd import ffi
+    prm = ffi.mc_params(seed=seed, max_nsteps=steps, steps_per_launch=5)
+    prm.mutation = 1
+    prm.trans_sigma = 1.0
+    prm.rot_sigma = 0.3
+    prm.relax = ffi.lbfgs_params(300, history=8, tol_grad=1e-3)
+    prm.full = ffi.lbfgs_params(300, history=8, tol_grad=1e-3)
+    return prm
+
+
+def search_leg(args, rec, box, R, G, workers):
+    import torch
+    from concurrent.futures import ProcessPoolExecutor
+
+    from pxdock_amd import lib, scoring, synth
+    with ProcessPoolExecutor(workers) as ex:
+        ligs = list(ex.map(_make_search_ligand, range(args.search_ligands), chunksize=8))
+    types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
+    Gs = G if set(types) <= set(G.types) else lib.Grid(R, box, types)
+    B = lib.LigandBatch(ligs)
+    sp = scoring.load_pose_selection()
+    x0 = [np.asarray(synth.ligand_apply(m.tree, m.xyz[0], rec.pocket_center, np.eye(3),
+                                        np.zeros(m.tree.n_torsions))) for m in ligs]
+    prm = search_params(7, args.search_steps)
+    lib.mc_search(R, Gs, lib.LigandBatch(ligs[:8]), x0[:8], sp, search_params(7, 1), box, 4)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    res = lib.mc_search(R, Gs, B, x0, sp, prm, box, args.search_walkers, num_modes=9,
+                        min_rmsd=0.5)
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    ne = res.nevals.cpu().numpy()
+    W = len(ligs) * args.search_walkers
+    # walker 0 of every ligand takes no Monte-Carlo step
+    steps = (W - len(ligs)) * args.search_steps
+    d = {"ligands": len(ligs), "walkers_per_ligand": args.search_walkers,
+         "steps_per_walker": args.search_steps, "seconds": dt,
+         "walker_steps_per_s": steps / dt, "evals": int(ne.sum()),
+         "evals_per_s": float(ne.sum() / dt),
+         "top9_found": float(res.cluster_count.cpu().numpy().mean()),
+         "params": "Gaussian mutation (1 A, 0.3 rad, one torsion), L-BFGS m=8, <=300 "
+                   "iterations, |g|inf <= 1e-3, T = 1.2, top-9 at 0.5 A; steps per chain cut "
+                   "from 2,000 to keep the run short"}
+    log(f"search: {d}")
+    if not args.no_cpu:
+        import pyoracle as O
+        from concurrent.futures import ThreadPoolExecutor
+        og = O.Grid(box, Gs.types, Gs.download().astype(np.float64), 1e6, rec.leaves.movable())
+        nw, ns = 32, 2
+        cprm = search_params(7, ns)
+        jobs = [(k % len(ligs), 1 + k // len(ligs)) for k in range(nw)]
+        systems = {l: O.System(rec, ligs[l], og) for l, _ in jobs}
+        threads = min(os.cpu_count() or 1, 16)
+        t1 = time.time()
+        with ThreadPoolExecutor(threads) as ex:
+            outs = list(ex.map(lambda j: O.mc_walker(systems[j[0]], ligs[j[0]], sp, cprm, box,
+                                                     j[1], j[0], x0[j[0]]), jobs))
+        dtc = time.time() - t1
+        d["cpu"] = {"walkers": nw, "steps_per_walker": ns, "seconds": dtc, "cores": threads,
+                    "walker_steps_per_s": nw * ns / dtc,
+                    "evals_per_s": float(sum(o["nevals"] for o in outs) / dtc)}
+        # per evaluation: the CPU sample's initial optimisations weigh more per step
+        d["speedup_evals_vs_cpu"] = d["evals_per_s"] / d["cpu"]["evals_per_s"]
+        log(f"search cpu: {d['cpu']}")
+    return d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +172,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true")
+    ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--search-ligands", type=int, default=256)
+    ap.add_argument("--search-walkers", type=int, default=64)
+    ap.add_argument("--search-steps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: sharding the path across GPUs (by ligand) "
@@ -257,6 +341,10 @@ def main():
             out["lbfgs"]["speedup_converged_vs_cpu"] = (
                 out["lbfgs"]["batched"]["converged_poses_per_s"] /
                 out["lbfgs"]["cpu"]["converged_poses_per_s"])
+    # ---------------- docking search: device MC walkers (bounded sample of the
+    # full-search workload: 256 ligands x 64 chains; steps cut from 2,000)
+    if not args.no_search:
+        out["search"] = search_leg(args, rec, box, R, G, workers)
     print(json.dumps(out), flush=True)
 
 
