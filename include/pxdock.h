@@ -201,6 +201,9 @@ typedef struct pxd_lbfgs_params {
   double ls_tol_change; /* 1e-9 */
   double gtd_threshold; /* 1e5 */
   int32_t max_ls;      /* 25 */
+  int32_t dof_mode;    /* 0: the reference's variables [ligand xyz | receptor torsions]
+                          (optimizer.cpp:56-76); 1: [translation | rotation vector |
+                          ligand torsions | receptor torsions] about the input pose */
 } pxd_lbfgs_params;
 
 /* Fill the reference defaults (optimizer.h:52-66) for the given iteration cap. */
@@ -375,6 +378,19 @@ pxd_status pxd_apply_poses(const pxd_ligand_batch* b, int32_t n_poses, const int
                            const int32_t* pose_atom_off, const float* xyz_in,
                            const float* center, const float* rot, const int32_t* tors_off,
                            const float* tors, float* xyz_out, void* stream);
+
+/* Gradient of the energy with respect to rigid-body + torsion variables,
+ * from the Cartesian ligand gradient of pxd_score_grad_batch (chain rule
+ * through free_ligand::apply_parameters, data.cpp:592-641):
+ * dof_grad[p][PXD_DOF_STRIDE] = force (3), torque about the pose's geometric
+ * centre (3), one torsion torque per ligand torsion in tree pre-order
+ * (zero-padded).  Receptor torsion gradients are pxd_score_grad_batch's.
+ * The reference optimises Cartesian coordinates and has no such gradient;
+ * this is the gradient of pxd_lbfgs_params.dof_mode 1 at zero variables. */
+#define PXD_DOF_STRIDE (6 + PXD_MAX_FRAGMENTS - 1)
+pxd_status pxd_dof_grad_batch(const pxd_ligand_batch* b, int32_t n_poses, const int32_t* pose_lig,
+                              const int32_t* pose_atom_off, const float* xyz,
+                              const float* lig_grad, float* dof_grad, void* stream);
 
 /* Batched pscore (root_scorer::combine on the pose_selection tree,
  * scorer.cpp:117-205 and leaves :207-613) for n_poses poses.  Receptor

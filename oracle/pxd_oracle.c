@@ -591,13 +591,21 @@ typedef struct {
   int n, nl3, nt;
   double *x0, *xt, *gtmp;
   int32_t* nevals;
+  int mode;            /* 0 Cartesian, 1 rigid-body + torsion */
+  const double* xyz0;  /* mode 1: input pose */
 } closure;
+
+/* the objective of either variable set at x */
+static double clo_eval(closure* c, const double* x, double* g) {
+  if (c->mode) return or_dof_energy(c->s, c->xyz0, x, g, NULL);
+  return or_system_energy(c->s, x + c->nl3, x, g + c->nl3, g);
+}
 
 /* obj_func_closure::evaluate_direction (optimizer.cpp:155-167):
  * f and g at x0 + t d, x0 unchanged */
 static double eval_dir(closure* c, double t, const double* d, double* g) {
   for (int i = 0; i < c->n; ++i) c->xt[i] = c->x0[i] + d[i] * t;
-  double e = or_system_energy(c->s, c->xt + c->nl3, c->xt, g + c->nl3, g);
+  double e = clo_eval(c, c->xt, g);
   if (g_noise_rel != 0.0) {
     e *= 1.0 + g_noise_rel * noise_u();
     for (int i = 0; i < c->n; ++i) g[i] *= 1.0 + g_noise_rel * noise_u();
@@ -755,7 +763,9 @@ static void strong_wolfe(closure* c, const pxd_lbfgs_params* p, double* t, const
  * [ligand xyz (3N) | receptor torsions (T)] (optimizer.cpp:56-76). */
 int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lx, double* tors,
                  double* energy, int32_t* nevals) {
-  int nl3 = 3 * s->lig->n_atoms;
+  const int mode = p->dof_mode ? 1 : 0;
+  if (mode && !s->tree) return 0;
+  int nl3 = mode ? 6 + s->tree->n_torsions : 3 * s->lig->n_atoms;
   int nt = s->leaves ? s->leaves->n_leaves : 0;
   int n = nl3 + nt, m = p->history;
   double* buf = (double*)calloc((size_t)n * (14 + 2 * (size_t)m) + 4 * (size_t)m, sizeof(double));
@@ -766,15 +776,22 @@ int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lx, doub
   double* ro = stps + (size_t)m * n;
   double* al = ro + m;
   int head = 0, num_old = 0; /* ring buffer of history */
-  memcpy(x, lx, sizeof(double) * nl3);
+  double* xyz0 = NULL;
+  if (mode) { /* variables start at zero: the input pose itself */
+    xyz0 = (double*)malloc(sizeof(double) * 3 * s->lig->n_atoms);
+    memcpy(xyz0, lx, sizeof(double) * 3 * s->lig->n_atoms);
+  } else {
+    memcpy(x, lx, sizeof(double) * nl3);
+  }
   if (nt) memcpy(x + nl3, tors, sizeof(double) * nt);
   *nevals = 0;
-  closure c = {s, n, nl3, nt, x, xt, NULL, nevals};
-  double loss = or_system_energy(s, x + nl3, x, g + nl3, g);
+  closure c = {s, n, nl3, nt, x, xt, NULL, nevals, mode, xyz0};
+  double loss = clo_eval(&c, x, g);
   *energy = loss;
   *nevals = 1;
   if (vamax(g, n) < p->tol_grad) {
     free(buf);
+    free(xyz0);
     return 1;
   }
   int high_change = 0, converged = 0, updated = 0, n_iter = 0;
@@ -858,13 +875,105 @@ int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lx, doub
     }
   }
   if (updated) {
-    *energy = or_system_energy(s, x + nl3, x, g + nl3, g);
+    *energy = clo_eval(&c, x, g);
     ++*nevals;
   }
-  memcpy(lx, x, sizeof(double) * nl3);
+  if (mode)
+    or_dof_energy(s, xyz0, x, g, lx); /* the final pose (energy already reported) */
+  else
+    memcpy(lx, x, sizeof(double) * nl3);
   if (nt) memcpy(tors, x + nl3, sizeof(double) * nt);
   free(buf);
+  free(xyz0);
   return converged;
+}
+
+/* ---- rigid-body + torsion variables ---- */
+
+void or_dof_grad(const pxd_frag_tree* t, int32_t n, const double* x, const double* g,
+                 const double center[3], double* out) {
+  double tq[PXD_MAX_FRAGMENTS][3];
+  memset(tq, 0, sizeof(tq));
+  out[0] = out[1] = out[2] = out[3] = out[4] = out[5] = 0.0;
+  for (int p = 0; p < n; ++p) {
+    const int a = t->perm[p];
+    const double* xa = x + 3 * a;
+    const double* ga = g + 3 * a;
+    double r[3], c[3];
+    sub3(xa, center, r);
+    cross3(r, ga, c);
+    for (int d = 0; d < 3; ++d) {
+      out[d] += ga[d];
+      out[3 + d] += c[d];
+    }
+    int f = 0;
+    while (f < t->n_frags && !(p >= t->frag_start[f] && p < t->frag_end[f])) ++f;
+    /* every torsion on the path to the root moves this atom */
+    for (; f < t->n_frags && t->frag_parent[f] >= 0; f = t->frag_parent[f]) {
+      const double* om = x + 3 * t->perm[t->frag_bond[2 * f + 1]];
+      sub3(xa, om, r);
+      cross3(r, ga, c);
+      axpy3(tq[f], 1.0, c);
+    }
+  }
+  for (int k = 0; k < t->n_torsions; ++k) {
+    int f = t->tors_frag[k];
+    double ax[3];
+    sub3(x + 3 * t->perm[t->frag_bond[2 * f + 1]], x + 3 * t->perm[t->frag_bond[2 * f]], ax);
+    scale3(ax, 1.0 / norm3(ax));
+    out[6 + k] = dot3(ax, tq[f]);
+  }
+}
+
+/* left Jacobian of SO(3) at w: exp([w + e]x) = exp([J e]x) exp([w]x) + O(e^2) */
+static void so3_left_jacobian(const double* w, double* J) {
+  double th2 = sqsum3(w), th = sqrt(th2), a, b;
+  if (th < 1e-4) {
+    a = 0.5 - th2 / 24.0;
+    b = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    a = (1.0 - cos(th)) / th2;
+    b = (th - sin(th)) / (th2 * th);
+  }
+  double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0}, K2[9];
+  or_matmul3(K, K, K2);
+  for (int i = 0; i < 9; ++i) J[i] = (i % 4 == 0 ? 1.0 : 0.0) + a * K[i] + b * K2[i];
+}
+
+double or_dof_energy(const or_system* s, const double* xyz0, const double* x, double* grad,
+                     double* pose_out) {
+  const pxd_frag_tree* t = s->tree;
+  const int n = s->lig->n_atoms, nlt = t->n_torsions;
+  double c0[3] = {0, 0, 0}, cen[3], R[9];
+  for (int a = 0; a < n; ++a) axpy3(c0, 1.0, xyz0 + 3 * a);
+  scale3(c0, 1.0 / n);
+  for (int d = 0; d < 3; ++d) cen[d] = c0[d] + x[d];
+  double th = norm3(x + 3);
+  if (th > 0.0) {
+    double k[3] = {x[3] / th, x[4] / th, x[5] / th};
+    or_rotmat_axis(k, th, R);
+  } else {
+    double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(R, I, sizeof(I));
+  }
+  double* pose = (double*)malloc(sizeof(double) * 6 * n);
+  double* gc = pose + 3 * n;
+  or_ligand_apply(t, n, xyz0, cen, R, x + 6, pose);
+  int nt = s->leaves ? s->leaves->n_leaves : 0;
+  double e = or_system_energy(s, x + 6 + nlt, pose, grad + 6 + nlt, gc);
+  double dof[6 + PXD_MAX_FRAGMENTS], J[9], tau[3];
+  or_dof_grad(t, n, pose, gc, cen, dof);
+  so3_left_jacobian(x + 3, J);
+  for (int i = 0; i < 3; ++i) tau[i] = dof[3 + i];
+  for (int i = 0; i < 3; ++i) {
+    grad[i] = dof[i];
+    grad[3 + i] = J[i] * tau[0] + J[3 + i] * tau[1] + J[6 + i] * tau[2];  /* J^T tau */
+  }
+  for (int k = 0; k < nlt; ++k) grad[6 + k] = dof[6 + k];
+  (void)nt;
+  if (pose_out) memcpy(pose_out, pose, sizeof(double) * 3 * n);
+  free(pose);
+  return e;
 }
 
 /* ---- threaded batch drivers (one pose per task, entrance.cpp:371-380) ---- */

@@ -85,6 +85,7 @@ typedef struct or_system {
   const pxd_leaves* leaves;
   const pxd_mol* lig;
   const or_grid* grid;       /* NULL: brute-force inter */
+  const pxd_frag_tree* tree; /* ligand fragment tree (rigid+torsion L-BFGS only) */
 } or_system;
 
 /* binding_system_interactions::put_gradients (interaction.cpp:271-298).
@@ -98,9 +99,28 @@ double or_system_energy(const or_system* s, const double* tors, const double* li
  * sees by (1 + rel * u), u uniform in [-1, 1) from a seeded stream, to model
  * FP32 evaluation error.  rel = 0 (the default) disables it.  Per thread. */
 void or_set_eval_noise(double rel, uint64_t seed);
-/* Optimises (lig_xyz, tors) in place; returns converged; energy/nevals out. */
+/* Optimises (lig_xyz, tors) in place; returns converged; energy/nevals out.
+ * p->dof_mode 0: the reference's variables, [ligand xyz | receptor torsions]
+ * (optimizer.cpp:56-76).  1: rigid-body + torsion variables
+ * [translation | rotation vector | ligand torsions | receptor torsions] about
+ * the input pose, mapped to coordinates by free_ligand::apply_parameters
+ * (data.cpp:592-641) with s->tree; the gradient is the chain rule of the
+ * Cartesian one (or_dof_grad, with the SO(3) left Jacobian for the rotation
+ * vector). */
 int32_t or_lbfgs(const or_system* s, const pxd_lbfgs_params* p, double* lig_xyz,
                  double* tors, double* energy, int32_t* nevals);
+
+/* Rigid-body + torsion gradient of a pose from its Cartesian gradient g:
+ * out[0..2] force sum(g), out[3..5] torque sum((x - center) x g),
+ * out[6 + k] torsion k (tree pre-order): axis_k . sum over the atoms the
+ * torsion moves of (x - x_mine) x g, axis_k the unit bond vector from the
+ * parent-side atom to the moving-side atom. */
+void or_dof_grad(const pxd_frag_tree* t, int32_t n_atoms, const double* xyz, const double* g,
+                 const double center[3], double* out);
+/* The energy of or_lbfgs's rigid-body + torsion variables x about xyz0 and its
+ * gradient; pose (may be NULL) receives the coordinates. */
+double or_dof_energy(const or_system* s, const double* xyz0, const double* x, double* grad,
+                     double* pose);
 
 /* Threaded batch forms for the CPU baseline: systems[] one per ligand. */
 void or_score_many(const or_system* systems, int32_t n_poses, const int32_t* pose_lig,

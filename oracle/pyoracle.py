@@ -66,6 +66,8 @@ def lib():
             "or_pose_cluster": (i, [i, i, f64p, f64p, i32p, i, d, i32p]),
             "or_set_eval_noise": (None, [d, C.c_uint64]),
             "or_mc_walker": (i, [P, P, P, P, P, i, i, f64p, f64p, f64p, f64p, f64p, i32p]),
+            "or_dof_grad": (None, [P, i, f64p, f64p, f64p, f64p]),
+            "or_dof_energy": (d, [P, f64p, f64p, f64p, f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -125,7 +127,7 @@ class OrGrid(C.Structure):
 
 class OrSystem(C.Structure):
     _fields_ = [("rec", C.c_void_p), ("rec_xyz", f64p), ("leaves", C.c_void_p),
-                ("lig", C.c_void_p), ("grid", C.c_void_p)]
+                ("lig", C.c_void_p), ("grid", C.c_void_p), ("tree", C.c_void_p)]
 
 
 class Grid:
@@ -178,9 +180,12 @@ class System:
         self.grid = grid
         self.n_leaves = receptor.leaves.n_leaves if receptor.leaves is not None else 0
         self.n_lig = ligand.n_atoms
+        self.tc = ligand.tree.to_c() if ligand.tree is not None else None
+        self.n_tors = ligand.tree.n_torsions if ligand.tree is not None else 0
         self.c = OrSystem(C.addressof(self.rc), dp(self.rxyz),
                           C.addressof(self.leaves) if self.leaves is not None else None,
-                          C.addressof(self.lc), C.addressof(grid.c) if grid else None)
+                          C.addressof(self.lc), C.addressof(grid.c) if grid else None,
+                          C.addressof(self.tc) if self.tc is not None else None)
 
     def energy(self, lig_xyz, tors=None):
         tors = f64(tors if tors is not None else np.zeros(self.n_leaves))
@@ -189,6 +194,15 @@ class System:
         tg = np.zeros(max(self.n_leaves, 1))
         e = lib().or_system_energy(byref(self.c), dp(tors), dp(lx), dp(tg), dp(g))
         return e, g, tg[:self.n_leaves]
+
+    def dof_energy(self, xyz0, x):
+        """Energy, gradient and pose of the rigid-body + torsion variables x."""
+        xx = f64(x)
+        g = np.zeros_like(xx)
+        pose = np.zeros((self.n_lig, 3))
+        e = lib().or_dof_energy(byref(self.c), dp(f64(xyz0).reshape(-1, 3)), dp(xx), dp(g),
+                                dp(pose))
+        return e, g, pose
 
     def lbfgs(self, lig_xyz, tors, params):
         lx = f64(lig_xyz).reshape(-1, 3).copy()

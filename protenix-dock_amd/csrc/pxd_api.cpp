@@ -232,6 +232,7 @@ void pxd_lbfgs_default_params(pxd_lbfgs_params* p, int32_t max_iter) {
   p->ls_tol_change = 1e-9;
   p->gtd_threshold = 1e5;
   p->max_ls = 25;
+  p->dof_mode = 0;
 }
 
 // grid.cpp:42-55
@@ -1000,6 +1001,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   memcpy(b->h_natoms, natoms.data(), sizeof(int) * nlig);
   memcpy(b->h_ntors, ntors.data(), sizeof(int) * nlig);
   memcpy(b->type_used, used, sizeof(used));
+  b->has_trees = trees ? 1 : 0;
   *out = b;
   return PXD_OK;
 }
@@ -1062,6 +1064,8 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
     return fail(PXD_ERR_RANGE, "lbfgs: history %d outside 1..%d", prm->history, kMaxHistory);
   if (prm->max_iter < 0 || prm->max_eval < 0 || prm->max_ls < 0)
     return fail(PXD_ERR_ARG, "lbfgs: negative iteration limits");
+  if (prm->dof_mode && !b->has_trees)
+    return fail(PXD_ERR_STATE, "lbfgs: rigid-body + torsion variables need fragment trees");
   pxd_status st = check_grid_types(g, b);
   if (st != PXD_OK) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -1081,8 +1085,9 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   if (!g_ws.counter) HIP_TRY(hipMalloc(&g_ws.counter, sizeof(int) * 64));
   HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int), s));
   LbfgsArgs a;
-  a.prm = {prm->history, prm->max_iter, prm->max_eval, prm->max_ls, prm->lr, prm->tol_grad,
-           prm->tol_change, prm->c1, prm->c2, prm->ls_tol_change, prm->gtd_threshold};
+  a.prm = {prm->history, prm->max_iter,  prm->max_eval,      prm->max_ls,
+           prm->lr,      prm->tol_grad,  prm->tol_change,    prm->c1,
+           prm->c2,      prm->ls_tol_change, prm->gtd_threshold, prm->dof_mode ? 1 : 0};
   a.n_poses = n_poses;
   a.pose_lig = pose_lig;
   a.pose_off = pose_atom_off;
@@ -1097,6 +1102,20 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   HIP_TRY(hipGetLastError());
   // the shared workspace may be reused by the next call: finish this one
   HIP_TRY(hipStreamSynchronize(s));
+  return PXD_OK;
+}
+
+pxd_status pxd_dof_grad_batch(const pxd_ligand_batch* b, int32_t n_poses, const int32_t* pose_lig,
+                              const int32_t* pose_atom_off, const float* xyz,
+                              const float* lig_grad, float* dof_grad, void* stream) {
+  if (!b || n_poses < 0) return fail(PXD_ERR_ARG, "dof_grad: bad argument");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !lig_grad || !dof_grad)
+    return fail(PXD_ERR_ARG, "dof_grad: null array");
+  if (!b->has_trees) return fail(PXD_ERR_STATE, "dof_grad: the batch has no fragment trees");
+  launch_dof_grad(b->dev, n_poses, pose_lig, pose_atom_off, xyz, lig_grad, dof_grad,
+                  static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
   return PXD_OK;
 }
 

@@ -170,10 +170,42 @@ struct LbfgsCtx {
   LigInfo li;
   PoseLds s;
   float* v;  // V_COUNT vectors of kWave*VPL floats
-  int n3, n;
+  int n3, n;  // n3: ligand variables (3 N, or 6 + n_tors), n: all variables
   int nevals;
+  // rigid-body + torsion variables: the input pose, its centre, LDS scratch
+  int mode;
+  float* xyz0;   // [kMaxAtoms*3]
+  float c0[3];
+  ApplyLds ap;
+  float* tq;     // [kMaxFrags*3]
+  float* dv;     // [6 + kMaxFrags] current variables / chain-rule output
   __device__ float* vec(int i) const { return v + i * (kWave * VPL); }
 };
+
+#define PXD_DOF_LDS_FLOATS \
+  (pxd::kMaxAtoms * 3 + PXD_APPLY_LDS_FLOATS + pxd::kMaxFrags * 3 + 6 + pxd::kMaxFrags)
+
+// Pose of the rigid-body + torsion variables held in c.dv (translation,
+// rotation vector, ligand torsions; receptor torsions already in c.s.tors):
+// free_ligand::apply_parameters (data.cpp:592-641) of the input pose about
+// its centre.  cen receives the rotation centre.
+template <int VPL>
+__device__ inline void dof_pose(LbfgsCtx<VPL>& c, float* cen) {
+  const int lane = lane_id();
+  const LigInfo& li = c.li;
+  cen[0] = c.c0[0] + c.dv[0];
+  cen[1] = c.c0[1] + c.dv[1];
+  cen[2] = c.c0[2] + c.dv[2];
+  float R[9];
+  rotvec_matrix(c.dv + 3, R);
+  for (int f = lane; f < li.n_frags; f += kWave) c.ap.th[f] = 0.f;
+  for (int k = lane; k < 3 * li.n_atoms; k += kWave) c.s.xyz[k] = c.xyz0[k];
+  __syncthreads();
+  const int* tf = c.L->tors_frag + li.frag_off;
+  for (int k = lane; k < li.n_tors; k += kWave) c.ap.th[tf[k]] = c.dv[6 + k];
+  __syncthreads();
+  apply_ligand(*c.L, li, c.s.xyz, cen, R, c.ap);
+}
 
 template <int VPL>
 __device__ inline double vdot(const float* a, const float* b) {
@@ -229,6 +261,34 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
   const int lane = lane_id();
   const float* x = c.vec(V_X);
   float tf = (float)t;
+  if (c.mode) {
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+      int e = lane + kWave * k;
+      float xv = x[e] + d[e] * tf;
+      if (e < c.n3)
+        c.dv[e] = xv;
+      else if (e < c.n)
+        c.s.tors[e - c.n3] = xv;
+    }
+    __syncthreads();
+    float cen[3], w[3] = {c.dv[3], c.dv[4], c.dv[5]};
+    dof_pose(c, cen);
+    double f = eval_pose(*c.R, *c.L, c.G, c.li, c.s);
+    dof_grad(*c.L, c.li, c.s.xyz, c.s.grad, cen, c.tq, c.dv);
+    float tau[3] = {c.dv[3], c.dv[4], c.dv[5]}, jw[3];
+    so3_jlt(w, tau, jw);
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+      int e = lane + kWave * k;
+      g[e] = e < 3 ? c.dv[e]
+                   : (e < 6 ? jw[e - 3]
+                            : (e < c.n3 ? c.dv[e] : (e < c.n ? c.s.torsg[e - c.n3] : 0.f)));
+    }
+    __syncthreads();
+    c.nevals++;
+    return f;
+  }
 #pragma unroll
   for (int k = 0; k < VPL; ++k) {
     int e = lane + kWave * k;
@@ -539,13 +599,14 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
   *converged_out = converged;
 }
 
-template <int VPL>
+template <int VPL, int MODE>
 __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
                                               const DevGrid* __restrict__ G, LbfgsArgs A) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
   __shared__ double s_al[kMaxHistory];
+  __shared__ __align__(16) float dofl[MODE ? PXD_DOF_LDS_FLOATS : 1];
   __shared__ int s_pose;
   const int lane = lane_id();
   constexpr int NV = kWave * VPL;
@@ -557,6 +618,13 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   c.G = G;
   c.s = carve_pose_lds(lds);
   c.v = vpool;
+  c.mode = MODE;
+  if (MODE) {
+    c.xyz0 = dofl;
+    c.ap = carve_apply_lds(dofl + kMaxAtoms * 3);
+    c.tq = dofl + kMaxAtoms * 3 + PXD_APPLY_LDS_FLOATS;
+    c.dv = c.tq + kMaxFrags * 3;
+  }
   float* x = c.vec(V_X);
   for (;;) {
     if (lane == 0) s_pose = atomicAdd(A.counter, 1);
@@ -566,26 +634,44 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
     if (pi >= A.n_poses) break;
     c.li = L.info[A.pose_lig[pi]];
     load_ligand_params(L, c.li, G, c.s);
-    c.n3 = 3 * c.li.n_atoms;
+    c.n3 = MODE ? 6 + c.li.n_tors : 3 * c.li.n_atoms;
     c.n = c.n3 + R.n_leaves;
     c.nevals = 0;
     const long long off = 3ll * A.pose_off[pi];
+    if (MODE) {
+      // variables start at zero: the input pose about its centre
+      for (int k = lane; k < 3 * c.li.n_atoms; k += kWave) c.xyz0[k] = A.xyz[off + k];
+      __syncthreads();
+      centroid(c.xyz0, c.li.n_atoms, c.c0);
+    }
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
       int e = lane + kWave * k;
-      x[e] = e < c.n3 ? A.xyz[off + e]
+      x[e] = e < c.n3 ? (MODE ? 0.f : A.xyz[off + e])
                       : (e < c.n ? A.tors[(long long)pi * R.n_leaves + (e - c.n3)] : 0.f);
     }
     double energy;
     bool converged;
     lbfgs_solve<VPL>(c, A.prm, hy, hs, s_ro, s_al, &energy, &converged);
+    if (MODE) {  // the pose of the final variables
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) {
+        int e = lane + kWave * k;
+        if (e < c.n3) c.dv[e] = x[e];
+      }
+      __syncthreads();
+      float cen[3];
+      dof_pose(c, cen);
+      for (int k = lane; k < 3 * c.li.n_atoms; k += kWave) A.xyz[off + k] = c.s.xyz[k];
+    }
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
       int e = lane + kWave * k;
-      if (e < c.n3)
-        A.xyz[off + e] = x[e];
-      else if (e < c.n)
+      if (e < c.n3) {
+        if (!MODE) A.xyz[off + e] = x[e];
+      } else if (e < c.n) {
         A.tors[(long long)pi * R.n_leaves + (e - c.n3)] = x[e];
+      }
     }
     if (lane == 0) {
       A.energy[pi] = energy;
@@ -598,7 +684,10 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
 
 void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st) {
-  hipLaunchKernelGGL(k_lbfgs<kLbfgsVpl>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+  if (a.prm.dof_mode)
+    hipLaunchKernelGGL((k_lbfgs<kLbfgsVpl, 1>), dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+  else
+    hipLaunchKernelGGL((k_lbfgs<kLbfgsVpl, 0>), dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
 }
 
 // ============================================================== K5: pscore
@@ -680,6 +769,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   c.G = G;
   c.s = carve_pose_lds(lds);
   c.v = vpool;
+  c.mode = 0;
   PscoreLds ps;
   ps.rc = rings;
   ps.rn = rings + 3 * kMaxRecRings;
@@ -1032,6 +1122,44 @@ void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
                        hipStream_t st) {
   if (R.n_leaves <= 0 || n <= 0) return;
   hipLaunchKernelGGL(k_leaf_table, dim3(n, R.n_leaves), dim3(kWave), 0, st, R, n, shift, out);
+}
+
+// ============================================================== rigid-body + torsion gradient
+// One wave per pose: force, torque about the pose's geometric centre and the
+// torsion torques from the Cartesian gradient (dof_grad).
+
+__global__ __launch_bounds__(64) void k_dof_grad(DevLigands L, int n_poses,
+                                                 const int* __restrict__ pose_lig,
+                                                 const int* __restrict__ pose_off,
+                                                 const float* __restrict__ xyz,
+                                                 const float* __restrict__ grad,
+                                                 float* __restrict__ out) {
+  __shared__ float sx[kMaxAtoms * 3];
+  __shared__ float sg[kMaxAtoms * 3];
+  __shared__ float tq[kMaxFrags * 3];
+  __shared__ float dv[6 + kMaxFrags];
+  const int p = blockIdx.x;
+  if (p >= n_poses) return;
+  const int lane = lane_id();
+  const LigInfo li = L.info[pose_lig[p]];
+  const long long off = 3ll * pose_off[p];
+  for (int k = lane; k < 3 * li.n_atoms; k += kWave) {
+    sx[k] = xyz[off + k];
+    sg[k] = grad[off + k];
+  }
+  __syncthreads();
+  float cen[3];
+  centroid(sx, li.n_atoms, cen);
+  dof_grad(L, li, sx, sg, cen, tq, dv);
+  constexpr int W = 6 + kMaxFrags - 1;
+  for (int k = lane; k < W; k += kWave) out[(long long)p * W + k] = k < 6 + li.n_tors ? dv[k] : 0.f;
+}
+
+void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, const int* pose_off,
+                     const float* xyz, const float* grad, float* out, hipStream_t st) {
+  if (n_poses <= 0) return;
+  hipLaunchKernelGGL(k_dof_grad, dim3(n_poses), dim3(kWave), 0, st, L, n_poses, pose_lig,
+                     pose_off, xyz, grad, out);
 }
 
 }  // namespace pxd

@@ -37,6 +37,7 @@ struct ScoreArgs {
 struct LbfgsParamsDev {
   int history, max_iter, max_eval, max_ls;
   double lr, tol_grad, tol_change, c1, c2, ls_tol_change, gtd_threshold;
+  int dof_mode;  // 0: [ligand xyz | receptor torsions]; 1: rigid-body + torsions
 };
 
 struct LbfgsArgs {
@@ -141,5 +142,9 @@ void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st);
 // theta = 2*pi*(i + shift)/n.
 void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
                        hipStream_t st);
+// Rigid-body + torsion gradient of poses from their Cartesian gradient:
+// out[p][6 + kMaxFrags - 1] = force, torque about the centre, torsion torques.
+void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, const int* pose_off,
+                     const float* xyz, const float* grad, float* out, hipStream_t st);
 
 }  // namespace pxd

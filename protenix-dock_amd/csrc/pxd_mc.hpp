@@ -229,6 +229,95 @@ __device__ inline void apply_ligand(const DevLigands& L, const LigInfo& li, floa
   __syncthreads();
 }
 
+// ---------------------------------------------------------------- rigid-body + torsion variables
+// Chain rule of a pose's Cartesian gradient g onto translation, rotation and
+// ligand torsions: out[0..2] = sum g, out[3..5] = sum (x - cen) x g, out[6 + k]
+// = axis_k . sum over the atoms torsion k moves of (x - x_mine) x g, axis_k the
+// unit bond vector parent-side atom -> moving-side atom in the current pose
+// (free_ligand::apply_parameters rotates the subtree of a fragment about it,
+// data.cpp:556-590).  tq: LDS scratch [kMaxFrags*3]; out: LDS [6 + kMaxFrags].
+__device__ inline void dof_grad(const DevLigands& L, const LigInfo& li, const float* xyz,
+                                const float* g, const float* cen, float* tq, float* out) {
+  const int lane = lane_id();
+  const int n = li.n_atoms, nf = li.n_frags;
+  const int* F = L.frag + 6 * li.frag_off;
+  const int* perm = L.perm + li.atom_off;
+  for (int k = lane; k < 3 * nf; k += kWave) tq[k] = 0.f;
+  __syncthreads();
+  float fs[3] = {0.f, 0.f, 0.f}, ts[3] = {0.f, 0.f, 0.f};
+  for (int p = lane; p < n; p += kWave) {
+    const int a = perm[p];
+    const float xa[3] = {xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2]};
+    const float ga[3] = {g[3 * a], g[3 * a + 1], g[3 * a + 2]};
+    float r[3] = {xa[0] - cen[0], xa[1] - cen[1], xa[2] - cen[2]}, c[3];
+    cross3(r, ga, c);
+    for (int d = 0; d < 3; ++d) {
+      fs[d] += ga[d];
+      ts[d] += c[d];
+    }
+    int f = 0;
+    while (f < nf - 1 && !(p >= F[6 * f] && p < F[6 * f + 1])) ++f;
+    for (; F[6 * f + 2] >= 0; f = F[6 * f + 2]) {
+      const int om = perm[F[6 * f + 4]];
+      float rr[3] = {xa[0] - xyz[3 * om], xa[1] - xyz[3 * om + 1], xa[2] - xyz[3 * om + 2]};
+      cross3(rr, ga, c);
+      atomicAdd(&tq[3 * f], c[0]);
+      atomicAdd(&tq[3 * f + 1], c[1]);
+      atomicAdd(&tq[3 * f + 2], c[2]);
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    fs[d] = wave_sum(fs[d]);
+    ts[d] = wave_sum(ts[d]);
+  }
+  if (lane == 0)
+    for (int d = 0; d < 3; ++d) {
+      out[d] = fs[d];
+      out[3 + d] = ts[d];
+    }
+  __syncthreads();
+  const int* tf = L.tors_frag + li.frag_off;
+  for (int k = lane; k < li.n_tors; k += kWave) {
+    const int f = tf[k];
+    const int am = perm[F[6 * f + 4]], ap = perm[F[6 * f + 3]];
+    float ax[3] = {xyz[3 * am] - xyz[3 * ap], xyz[3 * am + 1] - xyz[3 * ap + 1],
+                   xyz[3 * am + 2] - xyz[3 * ap + 2]};
+    float inv = 1.f / sqrtf(dot3(ax, ax));
+    out[6 + k] = (ax[0] * tq[3 * f] + ax[1] * tq[3 * f + 1] + ax[2] * tq[3 * f + 2]) * inv;
+  }
+  __syncthreads();
+}
+
+// J_l(w)^T tau for the rotation vector w (left Jacobian of SO(3)):
+// tau - a (w x tau) + b (w x (w x tau)), a = (1 - cos t)/t^2, b = (t - sin t)/t^3.
+__device__ inline void so3_jlt(const float* w, const float* tau, float* out) {
+  float t2 = dot3(w, w), a, b;
+  if (t2 < 1e-8f) {
+    a = 0.5f - t2 / 24.f;
+    b = 1.f / 6.f - t2 / 120.f;
+  } else {
+    float t = sqrtf(t2), sn, cs;
+    sincosf(t, &sn, &cs);
+    a = (1.f - cs) / t2;
+    b = (t - sn) / (t2 * t);
+  }
+  float c1[3], c2[3];
+  cross3(w, tau, c1);
+  cross3(w, c1, c2);
+  for (int d = 0; d < 3; ++d) out[d] = tau[d] - a * c1[d] + b * c2[d];
+}
+
+// Rotation matrix of the rotation vector w (identity at w = 0).
+__device__ inline void rotvec_matrix(const float* w, float* R) {
+  float th = sqrtf(dot3(w, w));
+  if (th > 0.f) {
+    float inv = 1.f / th;
+    rot_axis(w[0] * inv, w[1] * inv, w[2] * inv, th, R);
+  } else {
+    for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.f : 0.f;
+  }
+}
+
 // check_atom_in_box (grid.h:42-46) for every ligand atom / the centroid.
 struct BoxF {
   float lo[3], hi[3];

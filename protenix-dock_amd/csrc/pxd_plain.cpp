@@ -290,6 +290,8 @@ extern "C" pxd_status pxd_lbfgs_plain(const pxd_receptor* r, const pxd_grid* g,
                                       void* stream) {
   if (!r || !b || !prm || n_poses < 0 || (n_poses > 0 && (!pose_lig || !pose_atom_off || !xyz)))
     return pxd_set_error_(PXD_ERR_ARG, "lbfgs_plain: bad argument");
+  if (prm->dof_mode)
+    return pxd_set_error_(PXD_ERR_ARG, "lbfgs_plain: Cartesian variables only (dof_mode 0)");
   if (n_poses == 0) return PXD_OK;
   if (g) {
     for (int t = 0; t < PXD_NUM_VDW_TYPES; ++t)

@@ -20,7 +20,7 @@ namespace {
 
 LbfgsParamsDev to_dev(const pxd_lbfgs_params& p) {
   return {p.history, p.max_iter, p.max_eval, p.max_ls, p.lr, p.tol_grad,
-          p.tol_change, p.c1, p.c2, p.ls_tol_change, p.gtd_threshold};
+          p.tol_change, p.c1, p.c2, p.ls_tol_change, p.gtd_threshold, 0};
 }
 
 bool bad_lbfgs(const pxd_lbfgs_params& p) {
@@ -69,6 +69,9 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   if (r->n_leaves > 0 && !best_tors) return pxd_set_error_(PXD_ERR_ARG, "mc_search: best_tors required");
   if (bad_lbfgs(prm->relax) || bad_lbfgs(prm->full))
     return pxd_set_error_(PXD_ERR_RANGE, "mc_search: L-BFGS parameters out of range");
+  if (prm->relax.dof_mode || prm->full.dof_mode)
+    return pxd_set_error_(PXD_ERR_ARG, "mc_search: the search optimises Cartesian variables "
+                                       "(dof_mode 0)");
   if (prm->max_nsteps < 0 || prm->max_tries < 1 || prm->steps_per_launch < 1)
     return pxd_set_error_(PXD_ERR_ARG, "mc_search: bad step parameters");
   if (num_modes < 0 || num_modes > 64)

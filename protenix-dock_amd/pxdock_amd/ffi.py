@@ -105,11 +105,12 @@ class PxdLbfgsParams(C.Structure):
         ("history", C.c_int32), ("max_iter", C.c_int32), ("max_eval", C.c_int32),
         ("lr", C.c_double), ("tol_grad", C.c_double), ("tol_change", C.c_double),
         ("c1", C.c_double), ("c2", C.c_double), ("ls_tol_change", C.c_double),
-        ("gtd_threshold", C.c_double), ("max_ls", C.c_int32),
+        ("gtd_threshold", C.c_double), ("max_ls", C.c_int32), ("dof_mode", C.c_int32),
     ]
 
 
-def lbfgs_params(max_iter, history=100, max_eval=None, tol_grad=0.5, tol_change=1e-4):
+def lbfgs_params(max_iter, history=100, max_eval=None, tol_grad=0.5, tol_change=1e-4,
+                 dof_mode=0):
     """Reference defaults of lbfgs_step (optimizer.h:52-66)."""
     p = PxdLbfgsParams()
     p.history = history
@@ -123,6 +124,7 @@ def lbfgs_params(max_iter, history=100, max_eval=None, tol_grad=0.5, tol_change=
     p.ls_tol_change = 1e-9
     p.gtd_threshold = 1e5
     p.max_ls = 25
+    p.dof_mode = dof_mode
     return p
 
 

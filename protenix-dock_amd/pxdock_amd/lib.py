@@ -65,6 +65,7 @@ def load():
         "pxd_receptor_leaf_tables": (i, [P, i]),
         "pxd_receptor_info": (i, [P, i32p, i32p, f64p, f64p]),
         "pxd_mc_default_params": (None, [P]),
+        "pxd_dof_grad_batch": (i, [P, i, P, P, P, P, P, P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
     }
@@ -362,3 +363,17 @@ def apply_poses(batch, pose_lig, xyz_in, center, rot, tors_list):
                                  _dev(d_c), _dev(d_r), _dev(d_toff), _dev(d_t), _dev(out),
                                  _stream()))
     return out.cpu().numpy().reshape(-1, 3), off
+
+
+DOF_STRIDE = 6 + 32 - 1
+
+
+def dof_grad(poses, lig_grad):
+    """pxd_dof_grad_batch: [n_poses][DOF_STRIDE] force, torque about the pose
+    centre, torsion torques, from the Cartesian gradient of score_grad."""
+    import torch
+    out = torch.empty((poses.n_poses, DOF_STRIDE), dtype=torch.float32, device=poses.xyz.device)
+    check(load().pxd_dof_grad_batch(poses.batch.h, poses.n_poses, _dev(poses.d_lig),
+                                    _dev(poses.d_off), _dev(poses.xyz), _dev(lig_grad), _dev(out),
+                                    _stream()))
+    return out
