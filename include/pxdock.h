@@ -392,6 +392,28 @@ pxd_status pxd_dof_grad_batch(const pxd_ligand_batch* b, int32_t n_poses, const 
                               const int32_t* pose_atom_off, const float* xyz,
                               const float* lig_grad, float* dof_grad, void* stream);
 
+/* Scoring-function parameters differentiated by pxd_pscore_grad_batch, in
+ * this order: the nine term coefficients (PXD_TERM_* order), then
+ * vdw_sigma_scale[99], then vdw_epsilon_scale[99]. */
+#define PXD_NUM_SCORE_PARAMS (PXD_NUM_TERMS + 2 * PXD_NUM_VDW_TYPES)
+
+/* pscore of each pose (as pxd_pscore_batch) with its analytic derivatives:
+ * dscore_dxyz (FP32, the CSR layout of xyz) with respect to the ligand
+ * coordinates, and dscore_dparams[n_poses][PXD_NUM_SCORE_PARAMS] (FP64) with
+ * respect to the parameters above.  Derivatives of the piecewise leaves are
+ * those of the piece the pose is on (the ramps, the vdW clamp and the
+ * stacking cut-offs are not differentiable at their breakpoints).  A
+ * coefficient's derivative is its unweighted leaf value when the term is
+ * active (|coef| > 1e-5) and 0 otherwise.  The reference scores but never
+ * differentiates the score (scorer.cpp:117-724); this is the backward pass of
+ * the scoring function for fitting it on the GPU.  dscore_dxyz and
+ * dscore_dparams may be NULL. */
+pxd_status pxd_pscore_grad_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
+                                 const pxd_score_params* sp, int32_t n_poses,
+                                 const int32_t* pose_lig, const int32_t* pose_atom_off,
+                                 const float* xyz, const float* tors, double* score,
+                                 float* dscore_dxyz, double* dscore_dparams, void* stream);
+
 /* Batched pscore (root_scorer::combine on the pose_selection tree,
  * scorer.cpp:117-205 and leaves :207-613) for n_poses poses.  Receptor
  * coordinates come from applying tors to the receptor's leaves

@@ -1119,6 +1119,25 @@ pxd_status pxd_dof_grad_batch(const pxd_ligand_batch* b, int32_t n_poses, const 
   return PXD_OK;
 }
 
+pxd_status pxd_pscore_grad_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
+                                 const pxd_score_params* sp, int32_t n_poses,
+                                 const int32_t* pose_lig, const int32_t* pose_atom_off,
+                                 const float* xyz, const float* tors, double* score,
+                                 float* dscore_dxyz, double* dscore_dparams, void* stream) {
+  if (!r || !b || !sp) return fail(PXD_ERR_ARG, "pscore_grad: null argument");
+  if (n_poses < 0) return fail(PXD_ERR_ARG, "pscore_grad: n_poses < 0");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !score)
+    return fail(PXD_ERR_ARG, "pscore_grad: null pose array");
+  if (r->dev.n_ring5 + r->dev.n_ring6 > 256)
+    return fail(PXD_ERR_RANGE, "pscore_grad: receptor has more than 256 pi rings");
+  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, nullptr};
+  launch_pscore_grad(r->dev, b->dev, *sp, a, dscore_dxyz, dscore_dparams,
+                     static_cast<hipStream_t>(stream));
+  HIP_TRY(hipGetLastError());
+  return PXD_OK;
+}
+
 pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
                             const pxd_score_params* sp, int32_t n_poses, const int32_t* pose_lig,
                             const int32_t* pose_atom_off, const float* xyz, const float* tors,

@@ -5,6 +5,7 @@
 #include "pxd_device.hpp"
 #include "pxd_kernels.hpp"
 #include "pxd_pscore.hpp"
+#include "pxd_pscore_grad.hpp"
 #include "pxd_mc.hpp"
 #include "pxd_leaf.hpp"
 
@@ -732,6 +733,56 @@ void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_pa
   if (a.n_poses <= 0) return;
   hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
                      a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms);
+}
+
+// pscore with its derivatives (pxd_pscore_grad.hpp), one wavefront per pose.
+__global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
+                                                    pxd_score_params sp, int n_poses,
+                                                    const int* __restrict__ pose_lig,
+                                                    const int* __restrict__ pose_off,
+                                                    const float* __restrict__ xyz,
+                                                    const float* __restrict__ tors,
+                                                    double* __restrict__ score,
+                                                    float* __restrict__ dxyz,
+                                                    double* __restrict__ dparams) {
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
+  __shared__ float s_gx[kMaxAtoms * 3];
+  __shared__ double s_gp[PXD_NUM_SCORE_PARAMS];
+  PoseLds s = carve_pose_lds(lds);
+  PscoreLds ps;
+  ps.rc = rings;
+  ps.rn = rings + 3 * kMaxRecRings;
+  ps.lc = rings + 6 * kMaxRecRings;
+  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps.terms = nullptr;
+  PscoreGradLds g{s_gx, s_gp};
+  const int p = xcd_block(blockIdx.x, gridDim.x);
+  if (p >= n_poses) return;
+  const int lane = lane_id();
+  const LigInfo li = L.info[pose_lig[p]];
+  load_ligand_params(L, li, nullptr, s);
+  const long long off = 3ll * pose_off[p];
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.xyz[k] = xyz[off + k];
+  for (int k = lane; k < R.n_leaves; k += kWave)
+    s.tors[k] = tors ? tors[(long long)p * R.n_leaves + k] : 0.f;
+  __syncthreads();
+  receptor_torsion2xyz(R, s);
+  __syncthreads();
+  double v = pscore_pose_grad(R, L, li, sp, s, ps, g);
+  if (lane == 0) score[p] = v;
+  if (dxyz)
+    for (int k = lane; k < li.n_atoms * 3; k += kWave) dxyz[off + k] = s_gx[k];
+  if (dparams)
+    for (int k = lane; k < PXD_NUM_SCORE_PARAMS; k += kWave)
+      dparams[(long long)p * PXD_NUM_SCORE_PARAMS + k] = s_gp[k];
+}
+
+void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
+                        const PscoreArgs& a, float* dxyz, double* dparams, hipStream_t st) {
+  if (a.n_poses <= 0) return;
+  hipLaunchKernelGGL(k_pscore_grad, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, dxyz, dparams);
 }
 
 // ============================================================== K4: Monte-Carlo search

@@ -122,6 +122,8 @@ void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st);
 void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                    const PscoreArgs& a, hipStream_t st);
+void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
+                        const PscoreArgs& a, float* dxyz, double* dparams, hipStream_t st);
 void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
                int n_slots, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);

@@ -66,6 +66,7 @@ def load():
         "pxd_receptor_info": (i, [P, i32p, i32p, f64p, f64p]),
         "pxd_mc_default_params": (None, [P]),
         "pxd_dof_grad_batch": (i, [P, i, P, P, P, P, P, P]),
+        "pxd_pscore_grad_batch": (i, [P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
     }
@@ -377,3 +378,21 @@ def dof_grad(poses, lig_grad):
                                     _dev(poses.d_off), _dev(poses.xyz), _dev(lig_grad), _dev(out),
                                     _stream()))
     return out
+
+
+NUM_SCORE_PARAMS = ffi.NUM_TERMS + 2 * ffi.NUM_VDW_TYPES
+
+
+def pscore_grad(receptor, poses, params, with_dxyz=True, with_dparams=True):
+    """pxd_pscore_grad_batch: pscore per pose, d score / d ligand xyz (CSR like
+    poses.xyz) and d score / d parameters [n_poses][NUM_SCORE_PARAMS]."""
+    import torch
+    dev = poses.xyz.device
+    s = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    dx = torch.empty_like(poses.xyz) if with_dxyz else None
+    dp = torch.empty((poses.n_poses, NUM_SCORE_PARAMS), dtype=torch.float64, device=dev) \
+        if with_dparams else None
+    check(load().pxd_pscore_grad_batch(receptor.h, poses.batch.h, _ref(params), poses.n_poses,
+                                       _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
+                                       _dev(poses.tors), _dev(s), _dev(dx), _dev(dp), _stream()))
+    return s, dx, dp
