@@ -160,6 +160,95 @@ def search_leg(args, rec, box, R, G, workers):
     return d
 
 
+def _make_train_ligand(i):
+    from pxdock_amd import synth
+    rng = np.random.default_rng(40000 + i + 7919)
+    nh = int(rng.integers(20, 51))
+    nt = int(rng.integers(0, 16))
+    nr = int(rng.integers(0, 4))
+    return synth.make_ligand(40000 + i, n_heavy=nh, n_torsions=nt, n_rings=nr)
+
+
+def train_leg(args, rec, R, workers):
+    """Differentiation workload: pose scores of 16,384 poses (ligands of 20-50
+    heavy atoms) with their derivatives for every atom coordinate and the
+    207 scoring parameters, loss = sum w (S - E_target)^2, E_target ~ N(-8, 2);
+    one step = forward + backward through the autograd function."""
+    import torch
+    from concurrent.futures import ProcessPoolExecutor
+
+    from pxdock_amd import lib, scoring, train
+    n_lig, per = 256, max(1, args.train_poses // 256)
+    lo = rec.pocket_center - 6.0
+    hi = rec.pocket_center + 6.0
+    with ProcessPoolExecutor(workers) as ex:
+        ligs = list(ex.map(_make_train_ligand, range(n_lig), chunksize=8))
+        poses = list(ex.map(_make_poses, [(40000 + i, l, lo, hi, per) for i, l in enumerate(ligs)],
+                            chunksize=8))
+    B = lib.LigandBatch(ligs)
+    pose_lig = np.repeat(np.arange(n_lig), per)
+    P = lib.PoseSet(B, pose_lig, [p for arr in poses for p in arr], rec.leaves.n_leaves)
+    sp = scoring.load_pose_selection()
+    dev = P.xyz.device
+    theta = torch.tensor(train.params_to_vector(sp), device=dev, requires_grad=True)
+    xyz = P.xyz.clone().requires_grad_(True)
+    target = torch.tensor(np.random.default_rng(3).normal(-8.0, 2.0, P.n_poses), device=dev)
+
+    def step():
+        s = train.pose_score(xyz, theta, R, P, sp)
+        loss = ((s - target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    k = 5
+    ev0.record()
+    for _ in range(k):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / k
+    # kernel alone
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(k):
+        lib.pscore_grad(R, P, sp)
+    ev1.record()
+    torch.cuda.synchronize()
+    kms = ev0.elapsed_time(ev1) / k
+    d = {"poses": P.n_poses, "ligands": n_lig, "params": lib.NUM_SCORE_PARAMS,
+         "ms_per_step": ms, "poses_per_s": P.n_poses / (ms / 1e3),
+         "kernel_ms": kms, "kernel_poses_per_s": P.n_poses / (kms / 1e3),
+         "workload": "pscore + d/dxyz + d/dparams of 16,384 poses (256 ligands of 20-50 heavy "
+                     "atoms x 64 poses near the pocket), loss sum (S - E_t)^2, E_t ~ N(-8, 2); "
+                     "a step is forward + backward through the autograd function"}
+    if not args.no_cpu:
+        import pyoracle as O
+        from concurrent.futures import ThreadPoolExecutor
+        flat = P.xyz.detach().cpu().numpy().reshape(-1, 3).astype(np.float64)
+        offs = P.pose_atom_off
+        ns = 256
+        idx = np.linspace(0, P.n_poses - 1, ns).astype(int)
+        threads = min(os.cpu_count() or 1, 16)
+
+        def one(p):
+            lig = ligs[pose_lig[p]]
+            x = flat[offs[p]:offs[p] + lig.n_atoms]
+            return O.pscore(sp, rec, rec.xyz, lig, x)[0]
+        t0 = time.time()
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(one, idx))
+        dt = time.time() - t0
+        d["cpu_pscore_forward_poses_per_s"] = ns / dt
+        d["cpu_note"] = (f"FP64 oracle pscore forward only (no derivatives), {ns} poses, "
+                         f"{threads} threads")
+    log(f"train: {d}")
+    return d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lbfgs", action="store_true")
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--train-poses", type=int, default=16384)
     ap.add_argument("--search-ligands", type=int, default=256)
     ap.add_argument("--search-walkers", type=int, default=64)
     ap.add_argument("--search-steps", type=int, default=10)
@@ -345,6 +436,8 @@ def main():
     # full-search workload: 256 ligands x 64 chains; steps cut from 2,000)
     if not args.no_search:
         out["search"] = search_leg(args, rec, box, R, G, workers)
+    if not args.no_train:
+        out["train"] = train_leg(args, rec, R, workers)
     print(json.dumps(out), flush=True)
 
 

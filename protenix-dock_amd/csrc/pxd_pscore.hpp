@@ -131,10 +131,13 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
     auto one = [&](const float* pj, float qj, float2 lj) {
       for (int h = 0; h < 2; ++h) {
         if (!has[h]) continue;
-        float d = dist3(lx[h], pj);
-        if (do_coul) ec += (double)(safe_div(lq[h] * qj, d) * kCoulomb);
+        float dx = lx[h][0] - pj[0], dy = lx[h][1] - pj[1], dz = lx[h][2] - pj[2];
+        // 1/(d + eps) by one hardware reciprocal square root (saturating at
+        // 1/eps for d = 0, as safe_divide, math.h:176-178)
+        float rd = fminf(__builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz), 1e20f);
+        if (do_coul) ec += (double)(lq[h] * qj * rd * kCoulomb);
         if (do_vdw) {
-          float u = safe_div((ls[h] + lj.x) * 0.5f, d);
+          float u = (ls[h] + lj.x) * 0.5f * rd;
           float u2 = u * u;
           float u6 = u2 * u2 * u2;
           float e = (u6 * u6 - u6) * (le[h] * lj.y * 4.f);
@@ -142,11 +145,10 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
         }
       }
     };
-    for (int k = 0; k < R.n_fixed; ++k) {
-      int j = R.fixed[k];
-      float4 v = R.xyzq[j];
+    for (int k = 0; k < R.n_fixed; ++k) {  // compacted fixed atoms: one stream
+      float4 v = R.fixed_xyzq[k];
       float pj[3] = {v.x, v.y, v.z};
-      one(pj, v.w, R.lj[j]);
+      one(pj, v.w, R.fixed_lj[k]);
     }
     for (int m = 0; m < R.n_mov; ++m) {
       int j = R.mov_atom[m];

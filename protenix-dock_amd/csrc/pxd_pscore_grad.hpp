@@ -129,18 +129,18 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
       for (int h = 0; h < 2; ++h) {
         if (!has[h]) continue;
         float r[3] = {lx[h][0] - pj[0], lx[h][1] - pj[1], lx[h][2] - pj[2]};
-        float d = sqrtf(dot3(r, r));
-        float rinv = d > 0.f ? 1.f / d : 0.f;
+        // 1/(d + eps) as in the forward pass; 1/d for the direction (0 at d = 0)
+        float rd = fminf(__builtin_amdgcn_rsqf(dot3(r, r)), 1e20f);
+        float rinv = rd < 1e20f ? rd : 0.f;
         float dedd = 0.f;
         if (do_coul) {
-          float den = d + kEps;
-          float e = lq[h] * qj / den * kCoulomb;
+          float e = lq[h] * qj * rd * kCoulomb;
           ec += (double)e;
-          dedd += wc * (-e / den);
+          dedd += wc * (-e * rd);
         }
         if (do_vdw) {
           float sgm = (ls[h] + lj.x) * 0.5f;
-          float u = safe_div(sgm, d);
+          float u = sgm * rd;
           float u2 = u * u;
           float u6 = u2 * u2 * u2;
           float u12 = u6 * u6;
@@ -162,11 +162,10 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         gx[h][2] += r[2] * f;
       }
     };
-    for (int k = 0; k < R.n_fixed; ++k) {
-      int j = R.fixed[k];
-      float4 v = R.xyzq[j];
+    for (int k = 0; k < R.n_fixed; ++k) {  // compacted fixed atoms: one stream
+      float4 v = R.fixed_xyzq[k];
       float pj[3] = {v.x, v.y, v.z};
-      one(pj, v.w, R.lj[j]);
+      one(pj, v.w, R.fixed_lj[k]);
     }
     for (int m = 0; m < R.n_mov; ++m) {
       int j = R.mov_atom[m];
