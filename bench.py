@@ -67,6 +67,21 @@ def workload(n_lig, per_lig, workers):
     return rec, box, ligs, poses
 
 
+def profiled_traffic():
+    """HBM bytes per k_score launch on this workload as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) recorded in
+    the committed profile summary; None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_score_batched_262144_poses.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1]), path[len(ROOT) + 1:]
+    except OSError:
+        pass
+    return None, None
+
+
 def flops_bytes_per_pose(lig, n_excl):
     """Algorithmic work of one score+grad evaluation of a pose with N atoms
     (counting add/mul = 1, FMA = 2, div/sqrt/transcendental = 1): bytes
@@ -329,7 +344,7 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": Bt / t_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"], roof["traffic_source"] = profiled_traffic()
     roof["kernel"] = "k_score"
     roof["algorithmic_bytes_per_launch"] = Bt
     roof["algorithmic_flops_per_launch"] = F
