@@ -272,6 +272,13 @@ pxd_status pxd_grid_upload(const pxd_receptor* r, const pxd_box* box, const int3
 /* Copies the maps back to host FP32 in the upload layout. */
 pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats);
 pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types);
+/* Memory layout of the maps read by the lookups (receptor_cache::get,
+ * grid.cpp:197-303, gives identical results in both):
+ *   0  point-major, as built and as uploaded/downloaded (default);
+ *   1  every cell's 8 corner points stored together, so one lookup reads one
+ *      128-byte line per map; costs 8x the map memory, kept beside layout 0.
+ * Waits for the stream; the choice applies to later launches. */
+pxd_status pxd_grid_set_layout(pxd_grid* g, int32_t layout, void* stream);
 pxd_status pxd_grid_set_slope(pxd_grid* g, double slope);
 void pxd_grid_destroy(pxd_grid* g);
 

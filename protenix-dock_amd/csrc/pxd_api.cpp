@@ -671,6 +671,10 @@ static pxd_status grid_alloc(const pxd_receptor* r, const pxd_box* box, const in
   D.slope = (float)slope;
   D.P = P;
   D.maps = reinterpret_cast<float4*>(static_cast<char*>(g->mem) + kGridHeader);
+  D.layout = 0;
+  D.C = (long long)(box->npoints[0] - 1) * (box->npoints[1] - 1) * (box->npoints[2] - 1);
+  D.cmaps = nullptr;
+  g->cmem = nullptr;
   *out = g;
   return PXD_OK;
 }
@@ -757,6 +761,36 @@ pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats) {
   return PXD_OK;
 }
 
+pxd_status pxd_grid_set_layout(pxd_grid* g, int32_t layout, void* stream) {
+  if (!g) return fail(PXD_ERR_ARG, "grid layout: null grid");
+  if (layout != 0 && layout != 1) return fail(PXD_ERR_ARG, "grid layout: %d (0 or 1)", layout);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (layout == 1 && !g->cmem) {
+    size_t bytes = (size_t)(g->n_types + 1) * g->dev.C * 8 * sizeof(float4);
+    HIP_TRY(hipMalloc(&g->cmem, bytes));
+    launch_expand_cells(g->dev, g->n_types + 1, static_cast<float4*>(g->cmem), s);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) {
+      (void)hipFree(g->cmem);
+      g->cmem = nullptr;
+      return fail(PXD_ERR_HIP, "grid layout: %s", hipGetErrorString(he));
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(s));  // no launch may still read the old descriptor
+  g->dev.layout = layout;
+  g->dev.cmaps = static_cast<const float4*>(g->cmem);
+  pxd_status st = grid_sync_struct(g);
+  if (st != PXD_OK) return st;
+  if (layout == 0 && g->cmem) {
+    HIP_TRY(hipFree(g->cmem));
+    g->cmem = nullptr;
+    g->dev.cmaps = nullptr;
+    return grid_sync_struct(g);
+  }
+  return PXD_OK;
+}
+
 pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types) {
   if (!g) return fail(PXD_ERR_ARG, "grid info: null grid");
   if (box) *box = g->box;
@@ -773,6 +807,7 @@ pxd_status pxd_grid_set_slope(pxd_grid* g, double slope) {
 void pxd_grid_destroy(pxd_grid* g) {
   if (!g) return;
   (void)hipFree(g->mem);
+  if (g->cmem) (void)hipFree(g->cmem);
   delete g;
 }
 

@@ -15,22 +15,63 @@ namespace pxd {
 
 // ---------------------------------------------------------------- reductions
 
+// Wave-wide sums with DPP lane moves (no LDS traffic): pairs, quads, the
+// half-row and row mirrors give every lane its 16-lane row sum; the row
+// broadcasts of lanes 15 and 31 then accumulate the four rows into lane 63,
+// which is read back as a uniform value.  GFX9 DPP controls: quad_perm 0xB1
+// (xor 1) and 0x4E (xor 2), row_half_mirror 0x141, row_mirror 0x140,
+// row_bcast:15 0x142 (rows 1 and 3), row_bcast:31 0x143 (rows 2 and 3).
+template <int CTRL, int ROWS>
+__device__ inline float dpp_f(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+template <int CTRL, int ROWS>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ inline float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp_f<0xB1, 0xF>(v);
+  v += dpp_f<0x4E, 0xF>(v);
+  v += dpp_f<0x141, 0xF>(v);
+  v += dpp_f<0x140, 0xF>(v);
+  v += dpp_f<0x142, 0xA>(v);
+  v += dpp_f<0x143, 0xC>(v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
+  v += dpp_d<0xB1, 0xF>(v);
+  v += dpp_d<0x4E, 0xF>(v);
+  v += dpp_d<0x141, 0xF>(v);
+  v += dpp_d<0x140, 0xF>(v);
+  v += dpp_d<0x142, 0xA>(v);
+  v += dpp_d<0x143, 0xC>(v);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int CTRL, int ROWS>
+__device__ inline float dpp_fmax_src(float v) {  // disabled rows read -inf
+  return __int_as_float(__builtin_amdgcn_update_dpp((int)0xff800000u, __float_as_int(v), CTRL,
+                                                    ROWS, 0xF, false));
 }
 
 __device__ inline float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
-  return v;
+  v = fmaxf(v, dpp_fmax_src<0xB1, 0xF>(v));
+  v = fmaxf(v, dpp_fmax_src<0x4E, 0xF>(v));
+  v = fmaxf(v, dpp_fmax_src<0x141, 0xF>(v));
+  v = fmaxf(v, dpp_fmax_src<0x140, 0xF>(v));
+  v = fmaxf(v, dpp_fmax_src<0x142, 0xA>(v));
+  v = fmaxf(v, dpp_fmax_src<0x143, 0xC>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -631,16 +672,6 @@ __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float
     }
     f[d] = (float)(fd - (double)lo[d]);
   }
-  long long ny = G.np[1], nz = G.np[2];
-  long long base = ((long long)lo[0] * ny + lo[1]) * nz + lo[2];
-  long long off[8] = {base,
-                      base + ny * nz,
-                      base + nz,
-                      base + ny * nz + nz,
-                      base + 1,
-                      base + ny * nz + 1,
-                      base + nz + 1,
-                      base + ny * nz + nz + 1};
   float x0 = f[0], y0 = f[1], z0 = f[2], mx = 1.f - x0, my = 1.f - y0, mz = 1.f - z0;
   float w[8] = {mx * my * mz, x0 * my * mz, mx * y0 * mz, x0 * y0 * mz,
                 mx * my * z0, x0 * my * z0, mx * y0 * z0, x0 * y0 * z0};
@@ -652,14 +683,40 @@ __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float
     g[2] += gt[2] * G.slope;
     hit = false;
   }
-  float4 c = interp(G.maps, off, w);
+  // layout 0: point-major maps, the corners 1, nz, ny*nz points apart;
+  // layout 1: the cell's 8 corners contiguous (one 128-byte line per map)
+  const float4* m0;
+  const float4* m1;
+  long long off[8];
+  if (G.layout) {
+    const long long cell =
+        (((long long)lo[0] * (G.np[1] - 1) + lo[1]) * (G.np[2] - 1) + lo[2]) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) off[k] = cell + k;
+    m0 = G.cmaps;
+    m1 = G.cmaps + G.C * 8 * slot;
+  } else {
+    const long long ny = G.np[1], nz = G.np[2];
+    const long long base = ((long long)lo[0] * ny + lo[1]) * nz + lo[2];
+    off[0] = base;
+    off[1] = base + ny * nz;
+    off[2] = base + nz;
+    off[3] = base + ny * nz + nz;
+    off[4] = base + 1;
+    off[5] = base + ny * nz + 1;
+    off[6] = base + nz + 1;
+    off[7] = base + ny * nz + nz + 1;
+    m0 = G.maps;
+    m1 = G.maps + G.P * slot;
+  }
+  float4 c = interp(m0, off, w);
   *e += (double)(c.x * q);
   if (hit) {
     g[0] += c.y * q;
     g[1] += c.z * q;
     g[2] += c.w * q;
   }
-  float4 v = interp(G.maps + G.P * slot, off, w);
+  float4 v = interp(m1, off, w);
   *e += (double)v.x;
   if (hit) {
     g[0] += v.y;

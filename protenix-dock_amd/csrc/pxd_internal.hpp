@@ -155,6 +155,11 @@ struct DevGrid {
   long long P;                // points per map
   const float4* maps;         // [(n_types+1)][P], map 0 = Coulomb
   int type_slot[kMaxTypes];   // GAFF2 index -> map index (-1: none)
+  // layout 1: every cell's 8 corner points stored together (one 128-byte
+  // line per map and lookup), [(n_types+1)][C][8], corner c = x | y << 1 | z << 2
+  int layout;
+  long long C;                // cells per map: (nx-1)(ny-1)(nz-1)
+  const float4* cmaps;
 };
 
 }  // namespace pxd
@@ -176,6 +181,7 @@ struct pxd_receptor {
 
 struct pxd_grid {
   pxd::DevGrid dev;
+  void* cmem;                 // cell-corner maps of layout 1 (nullptr: not built)
   pxd_box box;
   int n_types;
   int32_t types[PXD_NUM_VDW_TYPES];

@@ -1213,4 +1213,30 @@ void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, cons
                      pose_off, xyz, grad, out);
 }
 
+// ============================================================== map layout 1
+// Copies every cell's 8 corner points of every map next to each other:
+// cm[(m*C + cell)*8 + c] = maps[m*P + corner c of cell], c = x | y<<1 | z<<2.
+__global__ __launch_bounds__(256) void k_expand_cells(const float4* __restrict__ maps, long long P,
+                                                      int nx, int ny, int nz, int n_maps,
+                                                      float4* __restrict__ cm) {
+  const long long C = (long long)(nx - 1) * (ny - 1) * (nz - 1);
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= C * 8 * n_maps) return;
+  const int c = (int)(t & 7);
+  const long long cell = (t >> 3) % C;
+  const int m = (int)((t >> 3) / C);
+  const int k = (int)(cell % (nz - 1));
+  const int j = (int)((cell / (nz - 1)) % (ny - 1));
+  const int i = (int)(cell / ((long long)(nz - 1) * (ny - 1)));
+  const long long p = ((long long)(i + (c & 1)) * ny + (j + ((c >> 1) & 1))) * nz + k + (c >> 2);
+  cm[t] = maps[(long long)m * P + p];
+}
+
+void launch_expand_cells(const DevGrid& G, int n_maps, float4* cm, hipStream_t st) {
+  const long long C = (long long)(G.np[0] - 1) * (G.np[1] - 1) * (G.np[2] - 1);
+  const long long total = C * 8 * n_maps;
+  hipLaunchKernelGGL(k_expand_cells, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     G.maps, G.P, G.np[0], G.np[1], G.np[2], n_maps, cm);
+}
+
 }  // namespace pxd

@@ -144,6 +144,8 @@ void launch_apply(const DevLigands& L, const ApplyArgs& a, hipStream_t st);
 // theta = 2*pi*(i + shift)/n.
 void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
                        hipStream_t st);
+// Map layout 1: every cell's 8 corners of every map stored together.
+void launch_expand_cells(const DevGrid& G, int n_maps, float4* cm, hipStream_t st);
 // Rigid-body + torsion gradient of poses from their Cartesian gradient:
 // out[p][6 + kMaxFrags - 1] = force, torque about the centre, torsion torques.
 void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, const int* pose_off,

@@ -54,6 +54,7 @@ def load():
         "pxd_grid_download": (i, [P, f32p, C.c_size_t]),
         "pxd_grid_info": (i, [P, P, i32p]),
         "pxd_grid_set_slope": (i, [P, d]),
+        "pxd_grid_set_layout": (i, [P, i, P]),
         "pxd_grid_destroy": (None, [P]),
         "pxd_ligand_batch_create": (i, [P, P, i, PP]),
         "pxd_ligand_batch_destroy": (None, [P]),
@@ -166,6 +167,10 @@ class Grid:
         out = np.empty(n, np.float32)
         check(load().pxd_grid_download(self.h, out.ctypes.data_as(f32p), n))
         return out.reshape(len(self.types) + 1, self.box.n_points(), 4)
+
+    def set_layout(self, layout):
+        """0: point-major maps; 1: each cell's 8 corners stored together."""
+        check(load().pxd_grid_set_layout(self.h, int(layout), _stream()))
 
     def set_slope(self, slope):
         check(load().pxd_grid_set_slope(self.h, slope))

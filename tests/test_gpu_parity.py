@@ -395,3 +395,20 @@ def test_gpu_lbfgs_rigid_torsion_matches_oracle(world, iters, need):
         print("  ", r)
     print(f"agree {agree}/{P.n_poses}")
     assert agree >= need * P.n_poses, (agree, P.n_poses)
+
+
+def test_gpu_map_layouts_give_identical_results(world):
+    """Map layout 1 (cell corners together) reads the same values in the same
+    order as layout 0: energies and gradients are bit-identical."""
+    pose_lig, xyz, tors = _poses(world, 6, 71)
+    B = lib.LigandBatch(world["ligs"])
+    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    e0, g0, t0 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+    world["G"].set_layout(1)
+    try:
+        e1, g1, t1 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+    finally:
+        world["G"].set_layout(0)
+    np.testing.assert_array_equal(e1, e0)
+    np.testing.assert_array_equal(g1, g0)
+    np.testing.assert_array_equal(t1, t0)
