@@ -278,6 +278,7 @@ def main():
     ap.add_argument("--no-lbfgs", action="store_true")
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--no-train", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--train-poses", type=int, default=16384)
     ap.add_argument("--search-ligands", type=int, default=256)
     ap.add_argument("--search-walkers", type=int, default=64)
@@ -298,16 +299,24 @@ def main():
         f"mean atoms {np.mean([l.n_atoms for l in ligs]):.1f}")
     types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
     R = lib.Receptor(rec)
+    B = lib.LigandBatch(ligs)
+    pose_lig = np.repeat(np.arange(len(ligs)), args.poses_per_ligand)
+    xyz_list = [p for arr in poses for p in arr]
+    P = lib.PoseSet(B, pose_lig, xyz_list, rec.leaves.n_leaves)
+    # autotune launch parameters on an evenly strided eighth of the poses
+    tune = None
+    if not args.no_tune:
+        sub = np.arange(0, len(pose_lig), 8)
+        Pt = lib.PoseSet(B, pose_lig[sub], [xyz_list[i] for i in sub], rec.leaves.n_leaves)
+        tune = lib.autotune(R, box, types, Pt).as_dict()
+        del Pt
+        log(f"autotune: {tune}")
     torch.cuda.synchronize()
     tg0 = time.time()
     G = lib.Grid(R, box, types)
     torch.cuda.synchronize()
     grid_ms = (time.time() - tg0) * 1e3
     grid_bytes = (len(types) + 1) * box.n_points() * 16
-    B = lib.LigandBatch(ligs)
-    pose_lig = np.repeat(np.arange(len(ligs)), args.poses_per_ligand)
-    xyz_list = [p for arr in poses for p in arr]
-    P = lib.PoseSet(B, pose_lig, xyz_list, rec.leaves.n_leaves)
     n_poses = P.n_poses
     log(f"{n_poses} poses resident on the device; grid {box.npts} x {len(types) + 1} maps "
         f"built in {grid_ms:.1f} ms")
@@ -362,6 +371,7 @@ def main():
                    "ligands": len(ligs), "poses_per_ligand": args.poses_per_ligand,
                    "poses": n_poses, "grid_points": box.npts, "maps": len(types) + 1},
         "roofline": roof,
+        "tuning": tune,
         "grid_build": {"ms": grid_ms, "output_GB_per_s": grid_bytes / (grid_ms / 1e3) / 1e9,
                        "maps": len(types) + 1, "points": box.n_points()},
     }

@@ -433,6 +433,33 @@ pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
                             const float* xyz, const float* tors, double* score,
                             double* terms, void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Autotuning of launch parameters (no counterpart in the reference, whose    */
+/* CPU path has no launch parameters).                                       */
+
+typedef struct pxd_tuning {
+  int32_t grid_types_per_pass; /* type maps per pass of the grid build: 4, 8, 16 */
+  int32_t map_layout;          /* lookup layout applied to new grids (pxd_grid_set_layout) */
+  int32_t lbfgs_slots_per_cu;  /* resident L-BFGS work slots per compute unit */
+  double grid_ms[3];           /* measured build times for 4, 8, 16 types per pass */
+  double score_ms[2];          /* measured score+grad times for layouts 0, 1 */
+  double lbfgs_ms[4];          /* measured L-BFGS times for 2, 4, 8, 16 slots per CU */
+} pxd_tuning;
+
+/* The process-wide choice in force (defaults: 8, 0, 8; timings zero). */
+void pxd_get_tuning(pxd_tuning* out);
+/* Sets the three choices (timings ignored). */
+pxd_status pxd_set_tuning(const pxd_tuning* t);
+/* Times the variants on the caller's receptor, box, types, ligands and poses
+ * (device arrays, left unchanged) and keeps the fastest of each: grid build
+ * types per pass, map layout for score+grad, L-BFGS slots per CU (40
+ * iterations from the given poses).  Results of every variant are identical;
+ * only their speed differs. */
+pxd_status pxd_autotune(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                        int32_t n_types, const pxd_ligand_batch* b, int32_t n_poses,
+                        const int32_t* pose_lig, const int32_t* pose_atom_off, const float* xyz,
+                        const float* tors, void* stream, pxd_tuning* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -194,6 +194,15 @@ pxd_status build_leaf_tables(pxd_receptor* r) {
   return PXD_OK;  // no table met the bound: direct evaluation
 }
 
+// Launch parameters chosen by pxd_autotune (or pxd_set_tuning).
+struct Tuning {
+  std::mutex mu;
+  int grid_tpp = 8;
+  int map_layout = 0;
+  int lbfgs_slots_per_cu = 8;
+};
+Tuning g_tune;
+
 struct LbfgsWorkspace {
   std::mutex mu;
   float* hist = nullptr;
@@ -715,7 +724,13 @@ pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32
   a.P = g->dev.P;
   a.maps = const_cast<float4*>(g->dev.maps);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  launch_grid_build(a, s);
+  int tpp, layout;
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    tpp = g_tune.grid_tpp;
+    layout = g_tune.map_layout;
+  }
+  launch_grid_build(a, tpp, s);
   he = hipGetLastError();
   if (he == hipSuccess) he = hipStreamSynchronize(s);
   (void)hipFree(d_tlj);
@@ -724,6 +739,7 @@ pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32
     return fail(PXD_ERR_HIP, "grid build: %s", hipGetErrorString(he));
   }
   st = grid_sync_struct(g);
+  if (st == PXD_OK && layout) st = pxd_grid_set_layout(g, layout, stream);
   if (st != PXD_OK) {
     pxd_grid_destroy(g);
     return st;
@@ -1107,7 +1123,12 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   int dev = 0, ncu = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  int n_slots = std::min(n_poses, std::max(1, ncu) * 8);
+  int spc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    spc = g_tune.lbfgs_slots_per_cu;
+  }
+  int n_slots = std::min(n_poses, std::max(1, ncu) * spc);
   size_t per_slot = (size_t)2 * prm->history * kWave * kLbfgsVpl;
   std::lock_guard<std::mutex> lock(g_ws.mu);
   if (g_ws.hist_floats < per_slot * n_slots) {
@@ -1187,6 +1208,176 @@ pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
   launch_pscore(r->dev, b->dev, *sp, a, static_cast<hipStream_t>(stream));
   HIP_TRY(hipGetLastError());
   return PXD_OK;
+}
+
+// ------------------------------------------------------------------ autotuning
+
+void pxd_get_tuning(pxd_tuning* out) {
+  if (!out) return;
+  memset(out, 0, sizeof(*out));
+  std::lock_guard<std::mutex> lk(g_tune.mu);
+  out->grid_types_per_pass = g_tune.grid_tpp;
+  out->map_layout = g_tune.map_layout;
+  out->lbfgs_slots_per_cu = g_tune.lbfgs_slots_per_cu;
+}
+
+pxd_status pxd_set_tuning(const pxd_tuning* t) {
+  if (!t) return fail(PXD_ERR_ARG, "tuning: null argument");
+  if (t->grid_types_per_pass != 4 && t->grid_types_per_pass != 8 && t->grid_types_per_pass != 16)
+    return fail(PXD_ERR_ARG, "tuning: types per pass %d (4, 8 or 16)", t->grid_types_per_pass);
+  if (t->map_layout != 0 && t->map_layout != 1)
+    return fail(PXD_ERR_ARG, "tuning: map layout %d (0 or 1)", t->map_layout);
+  if (t->lbfgs_slots_per_cu < 1 || t->lbfgs_slots_per_cu > 64)
+    return fail(PXD_ERR_ARG, "tuning: slots per CU %d (1..64)", t->lbfgs_slots_per_cu);
+  std::lock_guard<std::mutex> lk(g_tune.mu);
+  g_tune.grid_tpp = t->grid_types_per_pass;
+  g_tune.map_layout = t->map_layout;
+  g_tune.lbfgs_slots_per_cu = t->lbfgs_slots_per_cu;
+  return PXD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// milliseconds of fn() on stream s, best of reps after one warm-up
+template <class Fn>
+pxd_status time_best(hipStream_t s, int reps, Fn fn, double* ms) {
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  double best = 1e300;
+  pxd_status st = PXD_OK;
+  for (int r = 0; r <= reps && st == PXD_OK; ++r) {
+    if (hipEventRecord(a, s) != hipSuccess) st = PXD_ERR_HIP;
+    if (st == PXD_OK) st = fn();
+    if (st == PXD_OK && hipEventRecord(b, s) != hipSuccess) st = PXD_ERR_HIP;
+    if (st == PXD_OK && hipEventSynchronize(b) != hipSuccess) st = PXD_ERR_HIP;
+    float t = 0.f;
+    if (st == PXD_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) st = PXD_ERR_HIP;
+    if (st == PXD_OK && r > 0) best = std::min(best, (double)t);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (st == PXD_ERR_HIP) return fail(PXD_ERR_HIP, "autotune: event timing failed");
+  *ms = best;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+pxd_status pxd_autotune(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
+                        int32_t n_types, const pxd_ligand_batch* b, int32_t n_poses,
+                        const int32_t* pose_lig, const int32_t* pose_atom_off, const float* xyz,
+                        const float* tors, void* stream, pxd_tuning* out) {
+  if (!r || !box || !b || !out || n_poses <= 0 || !pose_lig || !pose_atom_off || !xyz)
+    return fail(PXD_ERR_ARG, "autotune: bad argument");
+  if (r->n_leaves > 0 && !tors) return fail(PXD_ERR_ARG, "autotune: tors required");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  pxd_tuning t;
+  pxd_get_tuning(&t);
+  // atoms spanned by the poses (for scratch copies)
+  std::vector<int32_t> hl(n_poses), ho(n_poses);
+  HIP_TRY(hipMemcpy(hl.data(), pose_lig, 4 * (size_t)n_poses, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(ho.data(), pose_atom_off, 4 * (size_t)n_poses, hipMemcpyDeviceToHost));
+  size_t atoms = 0;
+  for (int p = 0; p < n_poses; ++p) {
+    if (hl[p] < 0 || hl[p] >= b->n) return fail(PXD_ERR_ARG, "autotune: pose ligand out of range");
+    atoms = std::max(atoms, (size_t)ho[p] + (size_t)b->h_natoms[hl[p]]);
+  }
+  const int L = r->n_leaves;
+  pxd_status st = PXD_OK;
+  // 1. grid build: type maps per pass
+  const int tpps[3] = {4, 8, 16};
+  double best = 1e300;
+  for (int k = 0; k < 3; ++k) {
+    {
+      std::lock_guard<std::mutex> lk(g_tune.mu);
+      g_tune.grid_tpp = tpps[k];
+      g_tune.map_layout = 0;
+    }
+    st = time_best(s, 1, [&]() -> pxd_status {
+      pxd_grid* g = nullptr;
+      pxd_status e = pxd_grid_build(r, box, types, n_types, 1e6, stream, &g);
+      pxd_grid_destroy(g);
+      return e;
+    }, &t.grid_ms[k]);
+    if (st != PXD_OK) return st;
+    if (t.grid_ms[k] < best) {
+      best = t.grid_ms[k];
+      t.grid_types_per_pass = tpps[k];
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    g_tune.grid_tpp = t.grid_types_per_pass;
+  }
+  pxd_grid* g = nullptr;
+  st = pxd_grid_build(r, box, types, n_types, 1e6, stream, &g);
+  if (st != PXD_OK) return st;
+  // 2. score+grad: map layout
+  DevBuf e, gx, tg, x2, t2, ne, cv;
+  HIP_TRY(hipMalloc(&e.p, 8 * (size_t)n_poses));
+  HIP_TRY(hipMalloc(&gx.p, 12 * atoms));
+  HIP_TRY(hipMalloc(&tg.p, 4 * (size_t)n_poses * (L + 1)));
+  best = 1e300;
+  for (int lay = 0; lay < 2 && st == PXD_OK; ++lay) {
+    st = pxd_grid_set_layout(g, lay, stream);
+    if (st == PXD_OK)
+      st = time_best(s, 3, [&]() {
+        return pxd_score_grad_batch(r, g, b, n_poses, pose_lig, pose_atom_off, xyz, tors,
+                                    (double*)e.p, (float*)gx.p, (float*)tg.p, stream);
+      }, &t.score_ms[lay]);
+    if (st == PXD_OK && t.score_ms[lay] < best) {
+      best = t.score_ms[lay];
+      t.map_layout = lay;
+    }
+  }
+  if (st == PXD_OK) st = pxd_grid_set_layout(g, t.map_layout, stream);
+  // 3. L-BFGS: resident work slots per CU (relax step, 40 iterations)
+  const int spcs[4] = {2, 4, 8, 16};
+  if (st == PXD_OK) {
+    HIP_TRY(hipMalloc(&x2.p, 12 * atoms));
+    HIP_TRY(hipMalloc(&t2.p, 4 * (size_t)n_poses * (L + 1)));
+    HIP_TRY(hipMalloc(&ne.p, 4 * (size_t)n_poses));
+    HIP_TRY(hipMalloc(&cv.p, 4 * (size_t)n_poses));
+  }
+  pxd_lbfgs_params lp;
+  pxd_lbfgs_default_params(&lp, 40);
+  best = 1e300;
+  for (int k = 0; k < 4 && st == PXD_OK; ++k) {
+    {
+      std::lock_guard<std::mutex> lk(g_tune.mu);
+      g_tune.lbfgs_slots_per_cu = spcs[k];
+    }
+    st = time_best(s, 1, [&]() -> pxd_status {
+      if (hipMemcpyAsync(x2.p, xyz, 12 * atoms, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          (L > 0 && hipMemcpyAsync(t2.p, tors, 4 * (size_t)n_poses * L, hipMemcpyDeviceToDevice,
+                                   s) != hipSuccess))
+        return fail(PXD_ERR_HIP, "autotune: copy failed");
+      return pxd_lbfgs_batch(r, g, b, &lp, n_poses, pose_lig, pose_atom_off, (float*)x2.p,
+                             L > 0 ? (float*)t2.p : nullptr, (double*)e.p, (int32_t*)ne.p,
+                             (int32_t*)cv.p, stream);
+    }, &t.lbfgs_ms[k]);
+    if (st == PXD_OK && t.lbfgs_ms[k] < best) {
+      best = t.lbfgs_ms[k];
+      t.lbfgs_slots_per_cu = spcs[k];
+    }
+  }
+  pxd_grid_destroy(g);
+  if (st != PXD_OK) return st;
+  st = pxd_set_tuning(&t);
+  *out = t;
+  return st;
 }
 
 }  // extern "C"

@@ -25,6 +25,9 @@ __device__ inline int xcd_block(int b, int nb) {
 // point, Coulomb for a unit probe and LJ per ligand type, each with the probe
 // gradient, summed over the fixed (non-movable) receptor atoms.  One thread
 // per point, TG types per pass, receptor atoms staged through LDS tiles.
+// Distances and reciprocals use correctly rounded sqrt/division: a map value
+// or gradient is a sum of ~N_rec terms that cancel heavily near atoms, and
+// the build runs once per receptor.
 
 template <int TG>
 __global__ __launch_bounds__(256) void k_grid(const float4* __restrict__ atoms,
@@ -102,14 +105,24 @@ __global__ __launch_bounds__(256) void k_grid(const float4* __restrict__ atoms,
           make_float4(sat(acc[t][0]), sat(acc[t][1]), sat(acc[t][2]), sat(acc[t][3]));
 }
 
-void launch_grid_build(const GridBuildArgs& a, hipStream_t st) {
+template <int TG>
+static void grid_passes(const GridBuildArgs& a, hipStream_t st) {
   const int threads = 256;
   long long blocks = (a.P + threads - 1) / threads;
-  for (int t0 = 0, first = 1; t0 < a.n_types || first; t0 += 8, first = 0) {
-    hipLaunchKernelGGL(k_grid<8>, dim3((unsigned)blocks), dim3(threads), 0, st, a.atoms, a.alj,
+  for (int t0 = 0, first = 1; t0 < a.n_types || first; t0 += TG, first = 0) {
+    hipLaunchKernelGGL(k_grid<TG>, dim3((unsigned)blocks), dim3(threads), 0, st, a.atoms, a.alj,
                        a.n_atoms, a.tlj, a.n_types, t0, a.init[0], a.init[1], a.init[2], a.np[1],
                        a.np[2], a.spacing, a.P, a.maps, first);
   }
+}
+
+void launch_grid_build(const GridBuildArgs& a, int types_per_pass, hipStream_t st) {
+  if (types_per_pass <= 4)
+    grid_passes<4>(a, st);
+  else if (types_per_pass <= 8)
+    grid_passes<8>(a, st);
+  else
+    grid_passes<16>(a, st);
 }
 
 // ============================================================== K2: score+grad

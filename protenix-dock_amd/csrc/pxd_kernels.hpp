@@ -115,7 +115,8 @@ struct ClusterArgs {
   int* out_count;  // [n_ligands]
 };
 
-void launch_grid_build(const GridBuildArgs& a, hipStream_t st);
+// types_per_pass: 4, 8 or 16 type maps per pass over the receptor atoms
+void launch_grid_build(const GridBuildArgs& a, int types_per_pass, hipStream_t st);
 void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const ScoreArgs& a, hipStream_t st);
 void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,

@@ -154,6 +154,21 @@ def mc_params(seed=0, max_nsteps=200, relax_iters=40, full_iters=240, steps_per_
     return p
 
 
+class PxdTuning(C.Structure):
+    _fields_ = [
+        ("grid_types_per_pass", C.c_int32), ("map_layout", C.c_int32),
+        ("lbfgs_slots_per_cu", C.c_int32), ("grid_ms", C.c_double * 3),
+        ("score_ms", C.c_double * 2), ("lbfgs_ms", C.c_double * 4),
+    ]
+
+    def as_dict(self):
+        return {"grid_types_per_pass": self.grid_types_per_pass, "map_layout": self.map_layout,
+                "lbfgs_slots_per_cu": self.lbfgs_slots_per_cu,
+                "grid_ms": {k: v for k, v in zip((4, 8, 16), self.grid_ms)},
+                "score_ms": {k: v for k, v in zip((0, 1), self.score_ms)},
+                "lbfgs_ms": {k: v for k, v in zip((2, 4, 8, 16), self.lbfgs_ms)}}
+
+
 def ptr(arr, ctype):
     """Pointer to a contiguous numpy array (None for None/empty)."""
     if arr is None:

@@ -68,6 +68,9 @@ def load():
         "pxd_mc_default_params": (None, [P]),
         "pxd_dof_grad_batch": (i, [P, i, P, P, P, P, P, P]),
         "pxd_pscore_grad_batch": (i, [P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_get_tuning": (None, [P]),
+        "pxd_set_tuning": (i, [P]),
+        "pxd_autotune": (i, [P, P, i32p, i, P, i, P, P, P, P, P, P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
     }
@@ -401,3 +404,27 @@ def pscore_grad(receptor, poses, params, with_dxyz=True, with_dparams=True):
                                        _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
                                        _dev(poses.tors), _dev(s), _dev(dx), _dev(dp), _stream()))
     return s, dx, dp
+
+
+def get_tuning():
+    t = ffi.PxdTuning()
+    load().pxd_get_tuning(_ref(t))
+    return t
+
+
+def set_tuning(grid_types_per_pass=8, map_layout=0, lbfgs_slots_per_cu=8):
+    t = ffi.PxdTuning()
+    t.grid_types_per_pass = grid_types_per_pass
+    t.map_layout = map_layout
+    t.lbfgs_slots_per_cu = lbfgs_slots_per_cu
+    check(load().pxd_set_tuning(_ref(t)))
+
+
+def autotune(receptor, box, types, poses):
+    """pxd_autotune on these poses; returns the chosen PxdTuning."""
+    t = ffi.PxdTuning()
+    ty = np.ascontiguousarray(types, dtype=np.int32)
+    check(load().pxd_autotune(receptor.h, _ref(box), ty.ctypes.data_as(i32p), len(ty),
+                              poses.batch.h, poses.n_poses, _dev(poses.d_lig), _dev(poses.d_off),
+                              _dev(poses.xyz), _dev(poses.tors), _stream(), _ref(t)))
+    return t

@@ -5,6 +5,8 @@ import glob
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 from pxdock_amd import lib
 
@@ -39,3 +41,15 @@ def test_error_channel_and_host_helpers():
     b = lib.decide_grid_dims([1, 2, 3], [40.0] * 3, 0.375)
     assert b.npts == (109, 109, 109)  # 107 voxels rounded up to even (grid.cpp:78)
     assert abs(b.init_xyz[0] - (1 - 108 * 0.375 / 2)) < 1e-12
+
+
+def test_tuning_defaults_and_validation():
+    t = lib.get_tuning()
+    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu) == (8, 0, 8)
+    for bad in ({"grid_types_per_pass": 5}, {"map_layout": 2}, {"lbfgs_slots_per_cu": 0}):
+        with pytest.raises(lib.PxdError):
+            lib.set_tuning(**bad)
+    lib.set_tuning(16, 1, 4)
+    t = lib.get_tuning()
+    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu) == (16, 1, 4)
+    lib.set_tuning()

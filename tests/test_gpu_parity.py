@@ -412,3 +412,33 @@ def test_gpu_map_layouts_give_identical_results(world):
     np.testing.assert_array_equal(e1, e0)
     np.testing.assert_array_equal(g1, g0)
     np.testing.assert_array_equal(t1, t0)
+
+
+def test_gpu_grid_build_variants_identical_and_autotune(world):
+    """4, 8 or 16 type maps per pass build bit-identical maps; pxd_autotune
+    times every variant and keeps a valid choice."""
+    maps = []
+    try:
+        for tpp in (4, 8, 16):
+            lib.set_tuning(tpp, 0, 8)
+            maps.append(lib.Grid(world["R"], world["box"], world["types"]).download())
+    finally:
+        lib.set_tuning()
+    np.testing.assert_array_equal(maps[0], maps[1])
+    np.testing.assert_array_equal(maps[0], maps[2])
+    pose_lig, xyz, tors = _poses(world, 16, 81, spread=2.0)
+    B = lib.LigandBatch(world["ligs"])
+    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    x_before = P.xyz.cpu().numpy().copy()
+    try:
+        t = lib.autotune(world["R"], world["box"], world["types"], P)
+        d = t.as_dict()
+        print("\nautotune:", d)
+        assert d["grid_types_per_pass"] in (4, 8, 16) and d["map_layout"] in (0, 1)
+        assert d["lbfgs_slots_per_cu"] in (2, 4, 8, 16)
+        assert all(v > 0 for v in d["grid_ms"].values())
+        assert all(v > 0 for v in d["score_ms"].values())
+        assert all(v > 0 for v in d["lbfgs_ms"].values())
+        np.testing.assert_array_equal(P.xyz.cpu().numpy(), x_before)  # inputs untouched
+    finally:
+        lib.set_tuning()
