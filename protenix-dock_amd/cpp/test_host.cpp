@@ -1,0 +1,271 @@
+// test_host.cpp — tests of the C++ host layer (pxdock.hpp), written the way
+// the reference engine's gtests are (engine/utest/test_interaction.cpp,
+// test_optimizer.cpp, test_conformer.cpp): the same fixtures, the same
+// calls, the same expected values (tests/golden/gtest_vectors.json).
+//
+//   test_host GOLDEN_DIR --cpu     host-only cases (JSON readers, receptor torsions)
+//   test_host GOLDEN_DIR           every case, GPU ones included
+//   test_host GOLDEN_DIR --dump R L  print the parsed receptor / ligand summary
+//
+// GPU cases compare FP32 device results with FP64 expectations at
+// max(gtest tolerance, 1e-4 relative).
+#include <cmath>
+#include <cstdio>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "pxdock.hpp"
+
+using namespace pxdock;
+
+namespace {
+
+int g_fail = 0;
+std::string g_dir;
+
+#define EXPECT_NEAR(a, b, tol)                                                              \
+  do {                                                                                      \
+    double a_ = (a), b_ = (b), t_ = (tol);                                                  \
+    if (!(std::fabs(a_ - b_) <= t_)) {                                                      \
+      std::printf("  FAIL %s:%d: %s = %.9g, expected %.9g (tol %.3g)\n", __FILE__, __LINE__, \
+                  #a, a_, b_, t_);                                                          \
+      ++g_fail;                                                                             \
+    }                                                                                       \
+  } while (0)
+#define EXPECT_TRUE(c)                                                       \
+  do {                                                                       \
+    if (!(c)) {                                                              \
+      std::printf("  FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);            \
+      ++g_fail;                                                              \
+    }                                                                        \
+  } while (0)
+
+struct Case {
+  const char* name;
+  bool gpu;
+  std::function<void()> fn;
+};
+
+json golden() { return json::parse(detail::read_file(g_dir + "/gtest_vectors.json")); }
+std::string path(const json& v) { return g_dir + "/" + v.str(); }
+double tol_e(double tol, double e) { return std::max(tol, 1e-4 * std::max(std::fabs(e), 1.0)); }
+
+double expr(const std::string& s) {  // the few symbolic inputs of the gtests
+  const double pi = 3.141592653589793;
+  if (s == "pi/2") return pi / 2;
+  if (s == "pi") return pi;
+  if (s == "-pi") return -pi;
+  if (s == "-pi/2") return -pi / 2;
+  if (s == "-pi/3") return -pi / 3;
+  if (s == "0") return 0.0;
+  if (s == "sqrt(2)/4") return std::sqrt(2.0) / 4;
+  if (s == "-sqrt(6)/8 - sqrt(3)/4") return -std::sqrt(6.0) / 8 - std::sqrt(3.0) / 4;
+  if (s == "sqrt(2)/8 - 0.75") return std::sqrt(2.0) / 8 - 0.75;
+  if (s == "sqrt(6)/4") return std::sqrt(6.0) / 4;
+  if (s == "0.25 - sqrt(18)/8") return 0.25 - std::sqrt(18.0) / 8;
+  if (s == "sqrt(6)/8 + sqrt(3)/4") return std::sqrt(6.0) / 8 + std::sqrt(3.0) / 4;
+  if (s == "-sqrt(2)/2") return -std::sqrt(2.0) / 2;
+  if (s == "-sqrt(6)/4") return -std::sqrt(6.0) / 4;
+  throw std::invalid_argument("unknown expression " + s);
+}
+double num_or_expr(const json& v) { return v.kind() == json::STR ? expr(v.str()) : v.num(); }
+
+// ---------------------------------------------------------------- host-only
+
+void receptor_apply_parameters() {  // test_conformer.cpp:267-293
+  json g = golden()["receptor_apply"];
+  auto r = torsional_receptor::load(path(g["receptor_json"]));
+  std::vector<param_t> t;
+  for (const json& v : g["torsions"].items()) t.push_back(num_or_expr(v));
+  EXPECT_TRUE(r->num_torsions() == t.size());
+  molecule_pose out = r->apply_parameters(t);
+  const json& want = g["out_xyz"];
+  EXPECT_TRUE(out.size() == want.size());
+  for (size_t i = 0; i < out.size() && i < want.size(); ++i)
+    for (int d = 0; d < 3; ++d) EXPECT_NEAR(out[i].xyz[d], want[i][d].num(), g["tol"].num());
+}
+
+void json_readers_reject_bad_input() {  // data.cpp:234-244, 331-364
+  bool threw = false;
+  try {
+    json::parse("{\"a\": [1, 2,}");
+  } catch (const conf_error&) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  json lig = json::parse(detail::read_file(g_dir + "/ligand_3.json"));
+  EXPECT_TRUE(free_ligand::parse(lig)->num_atoms() == 3);
+  threw = false;
+  try {  // a vdW id outside the GAFF2 table
+    std::string text = detail::read_file(g_dir + "/ligand_3.json");
+    size_t at = text.find("FF_vdW_paraidx");
+    at = text.find("1019", at);
+    if (at != std::string::npos) text.replace(at, 4, "1200");
+    free_ligand::parse(json::parse(text));
+  } catch (const conf_error& e) {
+    threw = std::string(e.what()).find("VDW type") != std::string::npos;
+  }
+  EXPECT_TRUE(threw);
+}
+
+// ---------------------------------------------------------------- GPU
+
+void interaction_receptor_torsions() {  // test_interaction.cpp:280-454
+  json g = golden()["system_torsions"];
+  auto r = torsional_receptor::load(path(g["receptor_json"]));
+  auto l = free_ligand::load(path(g["ligand_json"]));
+  binding_system_interactions model(r, l);
+  std::vector<param_t> tg;
+  molecule_pose lg;
+  model.init_gradients(tg, lg);
+  for (const json& c : g["cases"].items()) {
+    std::vector<param_t> t;
+    for (const json& v : c["torsions_deg"].items()) t.push_back(v.num() / 180.0 * 3.141592653589793);
+    model.reset_gradients(tg, lg);
+    param_t e = model.put_gradients(t, l->get_pose(0), tg, lg);
+    EXPECT_NEAR(e, c["energy"].num(), tol_e(c["energy_tol"].num(), c["energy"].num()));
+    for (size_t k = 0; k < tg.size(); ++k)
+      EXPECT_NEAR(tg[k], c["tors_grad"][k].num(), std::max(c["tors_grad_tol"].num(), 1e-4));
+    // put_gradients adds: a second call doubles the gradients
+    std::vector<param_t> tg1 = tg;
+    model.put_gradients(t, l->get_pose(0), tg, lg);
+    for (size_t k = 0; k < tg.size(); ++k) EXPECT_NEAR(tg[k], 2.0 * tg1[k], 1e-9);
+  }
+}
+
+void optimizer_lbfgs() {  // test_optimizer.cpp:177-221, lbfgs_step(2, 12)
+  json g = golden()["lbfgs"];
+  auto r = torsional_receptor::load(path(g["receptor_json"]));
+  auto l = free_ligand::load(path(g["ligand_json"]));
+  binding_system_interactions model(r, l);
+  const json& c = g["cases"][1];
+  lbfgs_step step((size_t)c["max_iter"].integer(), (size_t)c["max_eval"].integer());
+  optimized_result out;
+  out.ligand_xyz = l->get_pose(0);
+  for (const json& v : c["torsions"].items()) out.torsions.push_back(v.num());
+  step.apply(model, out);
+  EXPECT_NEAR(out.energy, c["energy"].num(), 1e-4 * std::fabs(c["energy"].num()));
+  for (size_t k = 0; k < out.torsions.size(); ++k)
+    EXPECT_NEAR(out.torsions[k], c["out_torsions"][k].num(), 1e-3);
+  for (size_t i = 0; i < out.ligand_xyz.size(); ++i)
+    for (int d = 0; d < 3; ++d) EXPECT_NEAR(out.ligand_xyz[i].xyz[d], c["out_xyz"][i][d].num(), 1e-3);
+  // lbfgs_step(1, 8) from zero torsions descends from the start
+  const json& c0 = g["cases"][0];
+  std::vector<param_t> tg;
+  molecule_pose lg;
+  model.init_gradients(tg, lg);
+  param_t e0 = model.put_gradients(std::vector<param_t>(3, 0.0), l->get_pose(0), tg, lg);
+  lbfgs_step step0((size_t)c0["max_iter"].integer(), (size_t)c0["max_eval"].integer());
+  optimized_result out0;
+  out0.ligand_xyz = l->get_pose(0);
+  out0.torsions.assign(3, 0.0);
+  step0.apply(model, out0);
+  EXPECT_TRUE(out0.energy < e0);
+  EXPECT_TRUE(out0.nevals >= 1 && out0.nevals <= 8 + 25 + 1);
+}
+
+void conformer_apply_ligand() {  // test_conformer.cpp:295-378, mutate case
+  json g = golden()["ligand_apply"];
+  auto l = free_ligand::load(path(g["ligand_json"]));
+  const json& m = g["mutate"];
+  double rot[9];
+  for (int k = 0; k < 9; ++k) rot[k] = expr(m["rot_expr"][k].str());
+  std::vector<param_t> t;
+  for (const json& v : m["torsions"].items()) t.push_back(num_or_expr(v));
+  atom_position c{{m["center"][0].num(), m["center"][1].num(), m["center"][2].num()}};
+  molecule_pose out = l->apply_parameters(l->get_pose(0), c, rot, t);
+  const json& want = m["out_xyz"];
+  EXPECT_TRUE(out.size() == want.size());
+  for (size_t i = 0; i < out.size() && i < want.size(); ++i)
+    for (int d = 0; d < 3; ++d) EXPECT_NEAR(out[i].xyz[d], want[i][d].num(), 2e-4);
+}
+
+void search_and_score() {  // monte_carlo_searcher + pose_selection end to end
+  json g = golden()["lbfgs"];
+  auto r = torsional_receptor::load(path(g["receptor_json"]));
+  auto l = free_ligand::load(path(g["ligand_json"]));
+  pxd_score_params sp;
+  std::memset(&sp, 0, sizeof(sp));
+  for (int t = 0; t < PXD_NUM_VDW_TYPES; ++t) sp.vdw_sigma_scale[t] = sp.vdw_epsilon_scale[t] = 1.0;
+  sp.coul_coef = 1.0;  // a Coulomb-only pose score
+  pose_selection ps(sp);
+  const molecule_pose& x = l->get_pose(0);
+  box_config bc = describe_boundary(x[0].xyz[0], x[0].xyz[1], x[0].xyz[2], 12.0, 12.0, 12.0);
+  monte_carlo_searcher mc(ps, bc, 4, 10, 7);
+  auto res = mc.search(r, nullptr, {l}, 4, 2, 0.5);
+  EXPECT_TRUE(res.size() == 1 && res[0].walkers.size() == 4);
+  binding_system_interactions model(r, l);
+  for (const auto& w : res[0].walkers) {
+    EXPECT_TRUE(w.nevals > 0);
+    // the reported score is the score of the reported pose
+    param_t s = ps.combine(*r, w.torsions, model, w.ligand_xyz);
+    EXPECT_NEAR(s, w.pscore, 1e-6 * std::max(1.0, std::fabs(s)));
+  }
+  EXPECT_TRUE(!res[0].modes.empty() && res[0].modes.size() <= 2);
+  for (size_t k = 1; k < res[0].modes.size(); ++k)
+    EXPECT_TRUE(res[0].walkers[res[0].modes[k - 1]].pscore <= res[0].walkers[res[0].modes[k]].pscore);
+}
+
+void dump(const std::string& rp, const std::string& lp) {
+  auto r = torsional_receptor::load(rp);
+  auto l = free_ligand::load(lp);
+  pxd_mol rm = r->get_ffdata().view((int32_t)r->num_atoms());
+  pxd_mol lm = l->mol_view();
+  pxd_frag_tree t = l->tree_view();
+  double cs = 0.0;
+  for (int i = 0; i < lm.n_dihedrals * 6; ++i) cs += lm.dih_phase[i] * (i + 1) + lm.dih_k[i];
+  for (int i = 0; i < lm.n_angles; ++i) cs += lm.angle_t0[i];
+  std::printf("{\"rec_atoms\": %zu, \"rec_leaves\": %zu, \"rec_bonds\": %d, \"rec_pairs\": %d, "
+              "\"lig_atoms\": %zu, \"lig_poses\": %zu, \"lig_torsions\": %zu, \"lig_frags\": %d, "
+              "\"lig_dihedrals\": %d, \"lig_pairs14\": %d, \"lig_pairs\": %d, \"lig_tstrain\": %d, "
+              "\"lig_checksum\": %.12e, \"tors_frag\": [",
+              r->num_atoms(), r->num_torsions(), rm.n_bonds, rm.n_pairs, l->num_atoms(),
+              l->get_nposes(), l->num_torsions(), t.n_frags, lm.n_dihedrals, lm.n_pairs14,
+              lm.n_pairs, lm.n_tstrain, cs);
+  for (int k = 0; k < t.n_torsions; ++k) std::printf("%s%d", k ? ", " : "", t.tors_frag[k]);
+  std::printf("], \"perm\": [");
+  for (size_t k = 0; k < l->num_atoms(); ++k) std::printf("%s%d", k ? ", " : "", t.perm[k]);
+  std::printf("]}\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    std::printf("usage: test_host GOLDEN_DIR [--cpu | --dump RECEPTOR LIGAND]\n");
+    return 2;
+  }
+  g_dir = argv[1];
+  std::string mode = argc > 2 ? argv[2] : "";
+  if (mode == "--dump" && argc > 4) {
+    dump(argv[3], argv[4]);
+    return 0;
+  }
+  const bool cpu_only = mode == "--cpu";
+  std::vector<Case> cases = {
+      {"receptor_apply_parameters", false, receptor_apply_parameters},
+      {"json_readers_reject_bad_input", false, json_readers_reject_bad_input},
+      {"interaction_receptor_torsions", true, interaction_receptor_torsions},
+      {"optimizer_lbfgs", true, optimizer_lbfgs},
+      {"conformer_apply_ligand", true, conformer_apply_ligand},
+      {"search_and_score", true, search_and_score},
+  };
+  int run = 0;
+  for (const Case& c : cases) {
+    if (c.gpu && cpu_only) continue;
+    int before = g_fail;
+    std::printf("[ RUN  ] %s\n", c.name);
+    std::fflush(stdout);
+    try {
+      c.fn();
+    } catch (const std::exception& e) {
+      std::printf("  FAIL exception: %s\n", e.what());
+      ++g_fail;
+    }
+    std::printf("[ %s ] %s\n", g_fail == before ? " OK " : "FAIL", c.name);
+    ++run;
+  }
+  std::printf("%d cases, %d failures\n", run, g_fail);
+  return g_fail ? 1 : 0;
+}
