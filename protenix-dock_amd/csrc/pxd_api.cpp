@@ -843,6 +843,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   std::vector<int> frag, torsf, perm, sets, rings, tsmoff;
   std::vector<int2> dons;
   std::vector<uint4> pmask;
+  std::vector<uint32_t> bslot, aslot, dslot, gat;
   std::vector<int> natoms(nlig), ntors(nlig);
   uint8_t used[PXD_NUM_VDW_TYPES] = {0};
   for (int l = 0; l < nlig; ++l) {
@@ -903,6 +904,40 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
       dkp.insert(dkp.end(), kp, kp + 3);
     }
     if (bad) return fail(PXD_ERR_CONF, "ligand %d: dihedral periodicity outside 0..15", l);
+    // bonded-gradient gather tables (chunks of 64 terms, slots atom by atom)
+    I.gat_off = (int)gat.size();
+    auto chunks = [&](const int32_t* ids, int cnt, int w, std::vector<uint32_t>& slotw) {
+      for (int c0 = 0; c0 < cnt; c0 += kWave) {
+        const int c1 = std::min(cnt, c0 + kWave);
+        std::vector<int> num(n, 0), first(n, 0);
+        for (int t = c0; t < c1; ++t)
+          for (int r = 0; r < w; ++r) ++num[ids[w * t + r]];
+        int acc = 0;
+        for (int a = 0; a < n; ++a) {
+          if (num[a] > 255) return false;
+          first[a] = acc;
+          acc += num[a];
+        }
+        std::vector<int> next = first;
+        for (int t = c0; t < c1; ++t) {
+          uint32_t sw = 0;
+          for (int r = 0; r < w; ++r) sw |= uint32_t(next[ids[w * t + r]]++) << (8 * r);
+          slotw.push_back(sw);
+        }
+        for (int lane = 0; lane < kWave; ++lane) {
+          uint32_t g = 0;
+          for (int h = 0; h < 2; ++h) {
+            int a = lane + h * kWave;
+            if (a < n && num[a]) g |= (uint32_t(first[a]) | uint32_t(num[a]) << 8) << (16 * h);
+          }
+          gat.push_back(g);
+        }
+      }
+      return true;
+    };
+    if (!chunks(m.bond_ij, m.n_bonds, 2, bslot) || !chunks(m.angle_ijk, m.n_angles, 3, aslot) ||
+        !chunks(m.dih_ijkl, m.n_dihedrals, 4, dslot))
+      return fail(PXD_ERR_CONF, "ligand %d: an atom in over 255 of 64 consecutive bonded terms", l);
     I.pair_off = (int)pairs.size();
     I.n_pairs = m.n_pairs14 + m.n_pairs;
     for (int p = 0; p < m.n_pairs14; ++p)
@@ -1011,7 +1046,9 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
          o_dijkl = P.add(dijkl), o_dper = P.add(dper), o_dkp = P.add(dkp), o_pairs = P.add(pairs),
          o_frag = P.add(frag), o_torsf = P.add(torsf), o_perm = P.add(perm), o_sets = P.add(sets),
          o_rings = P.add(rings), o_dons = P.add(dons), o_tsc = P.add(tscoef),
-         o_tsm = P.add(tsmoff), o_tsi = P.add(tsijkl), o_pmask = P.add(pmask);
+         o_tsm = P.add(tsmoff), o_tsi = P.add(tsijkl), o_pmask = P.add(pmask),
+         o_bslot = P.add(bslot), o_aslot = P.add(aslot), o_dslot = P.add(dslot),
+         o_gat = P.add(gat);
   pxd_ligand_batch* b = new pxd_ligand_batch();
   hipError_t he = hipMalloc(&b->mem, P.host.size());
   if (he == hipSuccess) he = hipMemcpy(b->mem, P.host.data(), P.host.size(), hipMemcpyHostToDevice);
@@ -1045,6 +1082,10 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   D.ts_moff = dptr<int>(B, o_tsm);
   D.ts_ijkl = dptr<uint32_t>(B, o_tsi);
   D.pmask = dptr<uint4>(B, o_pmask);
+  D.bond_slot = dptr<uint32_t>(B, o_bslot);
+  D.angle_slot = dptr<uint32_t>(B, o_aslot);
+  D.dih_slot = dptr<uint32_t>(B, o_dslot);
+  D.gat = dptr<uint32_t>(B, o_gat);
   b->mem_bytes = P.host.size();
   b->n = nlig;
   b->h_natoms = new int[nlig];

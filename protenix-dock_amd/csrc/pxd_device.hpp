@@ -81,6 +81,45 @@ __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
 // math.h:176-178
 __device__ inline float safe_div(float a, float b) { return a / (b + kEps); }
 
+// Hardware forms for the bonded terms, whose inputs are FP32 geometry with
+// relative errors near 1e-7 already: the reciprocal and square root are the
+// 1-ulp V_RCP_F32 / V_SQRT_F32 (no correctly-rounded refinement); 1/(b + eps)
+// saturates at 1e20 for b = 0 as safe_divide does.
+__device__ inline float safe_rcp(float b) { return fminf(__builtin_amdgcn_rcpf(b + kEps), 1e20f); }
+__device__ inline float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// atan2(y, x) for y >= 0 (every caller passes a vector norm as y): reduced to
+// t = min(|x|, y) / max(|x|, y) in [0, 1], atan(t) = t P(t^2) with P of degree
+// 7, fit to atan on [0, 1] by least squares on 4,001 Chebyshev nodes with
+// iteratively reweighted residuals (max error 1.3e-7 rad evaluated in FP32),
+// then unfolded: pi/2 - r when y > |x|, pi - r when x < 0.
+__device__ inline float atan2_ypos(float y, float x) {
+  float ax = fabsf(x);
+  float mx = fmaxf(ax, y), mn = fminf(ax, y);
+  float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f;
+  float s = t * t;
+  float p = -0.00405454f;
+  p = fmaf(p, s, 0.02186289f);
+  p = fmaf(p, s, -0.05591225f);
+  p = fmaf(p, s, 0.09642195f);
+  p = fmaf(p, s, -0.1390863f);
+  p = fmaf(p, s, 0.19946566f);
+  p = fmaf(p, s, -0.33329862f);
+  p = fmaf(p, s, 0.99999934f);
+  float r = p * t;
+  r = y > ax ? 1.57079632679489662f - r : r;
+  return x < 0.f ? 3.14159265358979324f - r : r;
+}
+
+// sin and cos of x radians from V_SIN_F32 / V_COS_F32, which take their
+// argument in revolutions: x / 2 pi is reduced to [-1/2, 1/2] first.
+__device__ inline void sincos_hw(float x, float* s, float* c) {
+  float r = x * 0.159154943091895336f;
+  r -= rintf(r);
+  *s = __builtin_amdgcn_sinf(r);
+  *c = __builtin_amdgcn_cosf(r);
+}
+
 // math.cpp:62-94: Rodrigues matrix and d/dtheta
 __device__ inline void rot_axis_grad(float kx, float ky, float kz, float t, float* m,
                                      float* j) {
@@ -145,16 +184,16 @@ __device__ inline float angle_grad(const float* pi, const float* pj, const float
   float v1[3] = {pj[0] - pk[0], pj[1] - pk[1], pj[2] - pk[2]};
   float cr[3];
   cross3(v0, v1, cr);
-  float rp = sqrtf(dot3(cr, cr));
+  float rp = hw_sqrt(dot3(cr, cr));
   cross3(v0, cr, gi);
   cross3(cr, v1, gk);
-  float s0 = -safe_div(1.f, dot3(v0, v0) * rp);
-  float s1 = -safe_div(1.f, dot3(v1, v1) * rp);
+  float s0 = -safe_rcp(dot3(v0, v0) * rp);
+  float s1 = -safe_rcp(dot3(v1, v1) * rp);
   for (int d = 0; d < 3; ++d) {
     gi[d] *= s0;
     gk[d] *= s1;
   }
-  return atan2f(rp, dot3(v0, v1) + kEps);
+  return atan2_ypos(rp, dot3(v0, v1) + kEps);
 }
 
 // math.cpp:130-167
@@ -168,15 +207,15 @@ __device__ inline float dihedral_grad(const float* pi, const float* pj, const fl
   cross3(rij, rkj, m);
   cross3(rkj, rkl, n);
   cross3(m, n, w);
-  float wlen = sqrtf(dot3(w, w));
-  float phi = atan2f(wlen, dot3(m, n) + kEps);
+  float wlen = hw_sqrt(dot3(w, w));
+  float phi = atan2_ypos(wlen, dot3(m, n) + kEps);
   float theta = dot3(rij, n) < 0.f ? phi : -phi;
   float nrkj2 = dot3(rkj, rkj);
-  float nrkj_1 = safe_div(1.f, sqrtf(nrkj2));
+  float nrkj_1 = safe_rcp(hw_sqrt(nrkj2));
   float nrkj_2 = nrkj_1 * nrkj_1;
   float nrkj = nrkj2 * nrkj_1;
-  float a = -nrkj / dot3(m, m);
-  float b = nrkj / dot3(n, n);
+  float a = -nrkj * __builtin_amdgcn_rcpf(dot3(m, m));
+  float b = nrkj * __builtin_amdgcn_rcpf(dot3(n, n));
   float p = dot3(rij, rkj) * nrkj_2;
   float q = dot3(rkj, rkl) * nrkj_2;
   for (int d = 0; d < 3; ++d) {
@@ -201,7 +240,7 @@ __device__ inline float dihedral(const float* pi, const float* pj, const float* 
   cross3(rij, rkj, m);
   cross3(rkj, rkl, n);
   cross3(m, n, w);
-  float phi = atan2f(sqrtf(dot3(w, w)), dot3(m, n) + kEps);
+  float phi = atan2_ypos(hw_sqrt(dot3(w, w)), dot3(m, n) + kEps);
   return dot3(rij, n) < 0.f ? phi : -phi;
 }
 
@@ -233,7 +272,6 @@ __device__ inline float pair_energy(float rx, float ry, float rz, float qq, floa
 
 struct PoseLds {
   float* xyz;    // [kMaxAtoms*3]
-  float* grad;   // [kMaxAtoms*3]
   float* q;      // [kMaxAtoms]
   float2* lj;    // [kMaxAtoms] sigma, sqrt(eps)
   int* slot;     // [kMaxAtoms] grid map index
@@ -243,17 +281,21 @@ struct PoseLds {
   float* tors;   // [kMaxLeaves]
   float* torsg;  // [kMaxLeaves]
   float4* movp;  // [kMaxMovable] charge, sigma, sqrt(epsilon) of each movable slot
+  float* grad;   // [kMaxAtoms*3], overlaid by stage while the bonded terms run
+  float4* stage; // [kStageSlots] bonded-term gradient vectors of one term chunk
 };
 
-#define PXD_POSE_LDS_FLOATS                                                             \
-  (pxd::kMaxAtoms * 3 * 2 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 + \
-   pxd::kMaxLeaves * 2)
+constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
+
+#define PXD_POSE_LDS_FLOATS                                                   \
+  (pxd::kMaxAtoms * 3 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 +         \
+   pxd::kMaxLeaves * 2 +                                                      \
+   (pxd::kMaxAtoms * 3 > pxd::kStageSlots * 4 ? pxd::kMaxAtoms * 3 : pxd::kStageSlots * 4))
 
 __device__ inline PoseLds carve_pose_lds(float* base) {
   PoseLds s;
   s.xyz = base;
-  s.grad = s.xyz + kMaxAtoms * 3;
-  s.q = s.grad + kMaxAtoms * 3;
+  s.q = s.xyz + kMaxAtoms * 3;
   s.lj = reinterpret_cast<float2*>(s.q + kMaxAtoms);
   s.slot = reinterpret_cast<int*>(s.q + kMaxAtoms * 3);
   s.mov = s.q + kMaxAtoms * 4;
@@ -262,6 +304,8 @@ __device__ inline PoseLds carve_pose_lds(float* base) {
   s.tors = s.jac + kMaxMovable * 3;
   s.torsg = s.tors + kMaxLeaves;
   s.movp = reinterpret_cast<float4*>(s.torsg + kMaxLeaves);
+  s.grad = reinterpret_cast<float*>(s.movp + kMaxMovable);
+  s.stage = reinterpret_cast<float4*>(s.grad);
   return s;
 }
 
@@ -317,7 +361,7 @@ __device__ inline float fourier(uint32_t per, const float4* kp, float theta, flo
     if (pn == 0) break;
     float dl = theta * pn - ph[n];
     float sn, cs;
-    sincosf(dl, &sn, &cs);
+    sincos_hw(dl, &sn, &cs);
     e += k[n] * (1.f + cs);
     de += k[n] * pn * sn;
   }
@@ -369,10 +413,10 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_pos(R, s, ab.x, pa);
     rec_pos(R, s, ab.y, pb);
     float d[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
-    float r = sqrtf(dot3(d, d));
+    float r = hw_sqrt(dot3(d, d));
     float dl = r - kr.y;
     e += 0.5f * kr.x * dl * dl;
-    float de = kr.x * (1.f - safe_div(kr.y, r));
+    float de = kr.x * (1.f - kr.y * safe_rcp(r));
     rec_grad_add(s, ab.x, d, -de);
     rec_grad_add(s, ab.y, d, de);
   }
@@ -454,90 +498,153 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
   return e;
 }
 
+// Bonded-term gradients without atomics: a chunk of up to 64 terms (one per
+// lane) writes each (term, atom) gradient vector to its own stage slot; the
+// slots are numbered atom by atom, so lane then sums the contiguous slots of
+// atoms lane and lane + 64 into its registers g0 / g1 (fixed order: the sums
+// are deterministic).  gw: the chunk's gather word of this lane.
+__device__ inline void stage_gather(const PoseLds& s, uint32_t gw, float* g0, float* g1) {
+  __syncthreads();
+  const int o0 = gw & 255, c0 = (gw >> 8) & 255, o1 = (gw >> 16) & 255, c1 = gw >> 24;
+  for (int k = 0; k < c0; ++k) {
+    float4 v = s.stage[o0 + k];
+    g0[0] += v.x;
+    g0[1] += v.y;
+    g0[2] += v.z;
+  }
+  for (int k = 0; k < c1; ++k) {
+    float4 v = s.stage[o1 + k];
+    g1[0] += v.x;
+    g1[1] += v.y;
+    g1[2] += v.z;
+  }
+  __syncthreads();
+}
+
+__device__ inline void stage_put(const PoseLds& s, uint32_t slots, int r, const float* v, float f) {
+  s.stage[(slots >> (8 * r)) & 255] = make_float4(v[0] * f, v[1] * f, v[2] * f, 0.f);
+}
+
 // intra_molecular_interactions::put_gradients on the ligand (interaction.cpp:170-196):
-// bonds, angles and dihedrals by term lists (the first descriptors of every
-// list are loaded before any of them is used, later ones one iteration
-// ahead), non-bonded pairs by the list only for ligands without partner masks.
-__device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, PoseLds& s) {
+// bonds, angles and dihedrals by term chunks (descriptors of the next chunk
+// are loaded before the current one is evaluated), their gradients gathered
+// into g0 / g1; non-bonded pairs by the list (atomics into s.grad, which is
+// cleared here) only for ligands without partner masks.
+__device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, PoseLds& s,
+                                      float* g0, float* g1) {
   double e = 0.0;
   const int lane = lane_id();
   const uint32_t* bij = L.bond_ij + li.bond_off;
+  const uint32_t* bsl = L.bond_slot + li.bond_off;
   const float2* bkr = L.bond_kr + li.bond_off;
   const uint32_t* aijk = L.angle_ijk + li.angle_off;
+  const uint32_t* asl = L.angle_slot + li.angle_off;
   const float2* akt = L.angle_kt + li.angle_off;
   const uint32_t* dq = L.dih_ijkl + li.dih_off;
+  const uint32_t* dsl = L.dih_slot + li.dih_off;
   const uint32_t* dp = L.dih_per + li.dih_off;
   const float4* dkp = L.dih_kp + 3 * li.dih_off;
+  const uint32_t* gat = L.gat + li.gat_off + lane;
+  const int nbc = (li.n_bonds + kWave - 1) / kWave;
+  const int nac = (li.n_angles + kWave - 1) / kWave;
+  const int ndc = (li.n_dih + kWave - 1) / kWave;
   int b = lane, a = lane, t = lane;
-  uint32_t b_w = b < li.n_bonds ? bij[b] : 0u;
-  float2 b_k = b < li.n_bonds ? bkr[b] : make_float2(0.f, 0.f);
-  uint32_t a_w = a < li.n_angles ? aijk[a] : 0u;
-  float2 a_k = a < li.n_angles ? akt[a] : make_float2(0.f, 0.f);
-  uint32_t d_w = 0u, d_p = 0u;
+  uint32_t b_w = 0u, b_s = 0u, a_w = 0u, a_s = 0u, d_w = 0u, d_s = 0u, d_p = 0u;
+  float2 b_k = make_float2(0.f, 0.f), a_k = b_k;
   float4 d_k[3];
+  if (b < li.n_bonds) {
+    b_w = bij[b];
+    b_s = bsl[b];
+    b_k = bkr[b];
+  }
+  if (a < li.n_angles) {
+    a_w = aijk[a];
+    a_s = asl[a];
+    a_k = akt[a];
+  }
   if (t < li.n_dih) {
     d_w = dq[t];
+    d_s = dsl[t];
     d_p = dp[t];
     d_k[0] = dkp[3 * t];
     d_k[1] = dkp[3 * t + 1];
     d_k[2] = dkp[3 * t + 2];
   }
-  for (; b < li.n_bonds; b += kWave) {
-    uint32_t ij = b_w;
-    float2 kr = b_k;
+  int chunk = 0;
+  for (int c = 0; c < nbc; ++c, ++chunk, b += kWave) {
+    const uint32_t gw = gat[kWave * chunk];
+    const uint32_t ij = b_w, sl = b_s;
+    const float2 kr = b_k;
     if (b + kWave < li.n_bonds) {
       b_w = bij[b + kWave];
+      b_s = bsl[b + kWave];
       b_k = bkr[b + kWave];
     }
-    int i = ij & 255, j = (ij >> 8) & 255;
-    float d[3] = {s.xyz[3 * j] - s.xyz[3 * i], s.xyz[3 * j + 1] - s.xyz[3 * i + 1],
-                  s.xyz[3 * j + 2] - s.xyz[3 * i + 2]};
-    float r = sqrtf(dot3(d, d));
-    float dl = r - kr.y;
-    e += 0.5f * kr.x * dl * dl;
-    float de = kr.x * (1.f - safe_div(kr.y, r));
-    lig_grad_add(s, i, d, -de);
-    lig_grad_add(s, j, d, de);
+    if (b < li.n_bonds) {
+      int i = ij & 255, j = (ij >> 8) & 255;
+      float d[3] = {s.xyz[3 * j] - s.xyz[3 * i], s.xyz[3 * j + 1] - s.xyz[3 * i + 1],
+                    s.xyz[3 * j + 2] - s.xyz[3 * i + 2]};
+      float r = hw_sqrt(dot3(d, d));
+      float dl = r - kr.y;
+      e += 0.5f * kr.x * dl * dl;
+      float de = kr.x * (1.f - kr.y * safe_rcp(r));
+      stage_put(s, sl, 0, d, -de);
+      stage_put(s, sl, 1, d, de);
+    }
+    stage_gather(s, gw, g0, g1);
   }
-  for (; a < li.n_angles; a += kWave) {
-    uint32_t ijk = a_w;
-    float2 kt = a_k;
+  for (int c = 0; c < nac; ++c, ++chunk, a += kWave) {
+    const uint32_t gw = gat[kWave * chunk];
+    const uint32_t ijk = a_w, sl = a_s;
+    const float2 kt = a_k;
     if (a + kWave < li.n_angles) {
       a_w = aijk[a + kWave];
+      a_s = asl[a + kWave];
       a_k = akt[a + kWave];
     }
-    int i = ijk & 255, j = (ijk >> 8) & 255, k = (ijk >> 16) & 255;
-    float fi[3], fk[3];
-    float th = angle_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, fi, fk);
-    float dl = th - kt.y;
-    e += 0.5f * kt.x * dl * dl;
-    float de = kt.x * dl;
-    float fj[3] = {fi[0] + fk[0], fi[1] + fk[1], fi[2] + fk[2]};
-    lig_grad_add(s, i, fi, de);
-    lig_grad_add(s, k, fk, de);
-    lig_grad_add(s, j, fj, -de);
+    if (a < li.n_angles) {
+      int i = ijk & 255, j = (ijk >> 8) & 255, k = (ijk >> 16) & 255;
+      float fi[3], fk[3];
+      float th = angle_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, fi, fk);
+      float dl = th - kt.y;
+      e += 0.5f * kt.x * dl * dl;
+      float de = kt.x * dl;
+      float fj[3] = {fi[0] + fk[0], fi[1] + fk[1], fi[2] + fk[2]};
+      stage_put(s, sl, 0, fi, de);
+      stage_put(s, sl, 1, fj, -de);
+      stage_put(s, sl, 2, fk, de);
+    }
+    stage_gather(s, gw, g0, g1);
   }
-  for (; t < li.n_dih; t += kWave) {
-    uint32_t q = d_w, per = d_p;
+  for (int c = 0; c < ndc; ++c, ++chunk, t += kWave) {
+    const uint32_t gw = gat[kWave * chunk];
+    const uint32_t q = d_w, sl = d_s, per = d_p;
     float4 kp[3] = {d_k[0], d_k[1], d_k[2]};
     if (t + kWave < li.n_dih) {
       d_w = dq[t + kWave];
+      d_s = dsl[t + kWave];
       d_p = dp[t + kWave];
       d_k[0] = dkp[3 * (t + kWave)];
       d_k[1] = dkp[3 * (t + kWave) + 1];
       d_k[2] = dkp[3 * (t + kWave) + 2];
     }
-    int i = q & 255, j = (q >> 8) & 255, k = (q >> 16) & 255, l = (q >> 24) & 255;
-    float g[4][3];
-    float th = dihedral_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, s.xyz + 3 * l, g[0],
-                             g[1], g[2], g[3]);
-    float dedt;
-    e += fourier(per, kp, th, &dedt);
-    lig_grad_add(s, i, g[0], -dedt);
-    lig_grad_add(s, j, g[1], -dedt);
-    lig_grad_add(s, k, g[2], -dedt);
-    lig_grad_add(s, l, g[3], -dedt);
+    if (t < li.n_dih) {
+      int i = q & 255, j = (q >> 8) & 255, k = (q >> 16) & 255, l = (q >> 24) & 255;
+      float g[4][3];
+      float th = dihedral_grad(s.xyz + 3 * i, s.xyz + 3 * j, s.xyz + 3 * k, s.xyz + 3 * l,
+                               g[0], g[1], g[2], g[3]);
+      float dedt;
+      e += fourier(per, kp, th, &dedt);
+      stage_put(s, sl, 0, g[0], -dedt);
+      stage_put(s, sl, 1, g[1], -dedt);
+      stage_put(s, sl, 2, g[2], -dedt);
+      stage_put(s, sl, 3, g[3], -dedt);
+    }
+    stage_gather(s, gw, g0, g1);
   }
+  // the stage is free again: clear the gradient array it overlays
+  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.grad[k] = 0.f;
+  __syncthreads();
   if (li.pmode) return e;  // pairs by partner masks (ligand_pairs_masked)
   for (int p = lane; p < li.n_pairs; p += kWave) {
     uint32_t w = L.pairs[li.pair_off + p];
@@ -827,14 +934,13 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
 __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                                    const LigInfo& li, PoseLds& s) {
   const int lane = lane_id();
-  for (int k = lane; k < li.n_atoms * 3; k += kWave) s.grad[k] = 0.f;
   receptor_torsion2xyz(R, s);
   __syncthreads();
   double e = receptor_intra(R, s);
   if (lane == 0) e += R.e_const;
   __syncthreads();
-  e += ligand_intra(L, li, s);
   float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
+  e += ligand_intra(L, li, s, g0, g1);
   if (li.pmode) e += ligand_pairs_masked(L, li, s, g0, g1);
   e += inter(R, G, li, s, g0, g1);
   __syncthreads();

@@ -113,6 +113,7 @@ struct LigInfo {
   int n_rot;
   int tsm_off;   // first entry of this ligand in ts_moff (n_ts + 1 entries)
   int pmode;     // 1: non-bonded pairs as per-atom partner masks (pmask), 0: pair list
+  int gat_off;   // first gather word of this ligand in gat (64 per term chunk)
 };
 
 struct DevLigands {
@@ -143,6 +144,17 @@ struct DevLigands {
   // non-bonded partner masks, [atom_off + a][2]: bit b of word b/32 set when
   // (a, b) is an all-pairs entry (.x..w of [0]) or a 1-4 entry ([1])
   const uint4* pmask;
+  // bonded-gradient gather: the terms of a list are taken 64 at a time (one
+  // per lane); every (term, atom) incidence of a chunk owns one float4 stage
+  // slot, numbered atom by atom so an atom's slots are contiguous.
+  // *_slot[t] holds the slot of each of the term's atoms (8 bits each, in
+  // term order); gat[gat_off + 64 c + lane] holds first slot | count << 8 of
+  // atom lane and (<< 16) of atom lane + 64 in chunk c (bond chunks, then
+  // angle chunks, then dihedral chunks).
+  const uint32_t* bond_slot;
+  const uint32_t* angle_slot;
+  const uint32_t* dih_slot;
+  const uint32_t* gat;
 };
 
 struct DevGrid {
