@@ -1180,7 +1180,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
     g_ws.hist_floats = per_slot * n_slots;
   }
   if (!g_ws.counter) HIP_TRY(hipMalloc(&g_ws.counter, sizeof(int) * 64));
-  HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int), s));
+  HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int) * kLbfgsClasses, s));
   LbfgsArgs a;
   a.prm = {prm->history, prm->max_iter,  prm->max_eval,      prm->max_ls,
            prm->lr,      prm->tol_grad,  prm->tol_change,    prm->c1,

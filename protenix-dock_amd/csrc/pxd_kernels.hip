@@ -185,6 +185,7 @@ struct LbfgsCtx {
   PoseLds s;
   float* v;  // V_COUNT vectors of kWave*VPL floats
   int n3, n;  // n3: ligand variables (3 N, or 6 + n_tors), n: all variables
+  int nk;     // vector elements per lane in use: ceil(n / 64) <= VPL
   int nevals;
   // rigid-body + torsion variables: the input pose, its centre, LDS scratch
   int mode;
@@ -221,29 +222,33 @@ __device__ inline void dof_pose(LbfgsCtx<VPL>& c, float* cen) {
   apply_ligand(*c.L, li, c.s.xyz, cen, R, c.ap);
 }
 
+// Vector algebra over the nk elements per lane in use (nk is wave-uniform).
 template <int VPL>
-__device__ inline double vdot(const float* a, const float* b) {
+__device__ inline double vdot(int nk, const float* a, const float* b) {
   const int lane = lane_id();
   double acc = 0.0;
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) acc += (double)a[lane + kWave * k] * (double)b[lane + kWave * k];
+  for (int k = 0; k < VPL; ++k)
+    if (k < nk) acc += (double)a[lane + kWave * k] * (double)b[lane + kWave * k];
   return wave_sum(acc);
 }
 
 template <int VPL>
-__device__ inline float vamax(const float* a) {
+__device__ inline float vamax(int nk, const float* a) {
   const int lane = lane_id();
   float m = 0.f;
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) m = fmaxf(m, fabsf(a[lane + kWave * k]));
+  for (int k = 0; k < VPL; ++k)
+    if (k < nk) m = fmaxf(m, fabsf(a[lane + kWave * k]));
   return wave_max(m);
 }
 
 template <int VPL>
-__device__ inline void vcopy(float* dst, const float* src) {
+__device__ inline void vcopy(int nk, float* dst, const float* src) {
   const int lane = lane_id();
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) dst[lane + kWave * k] = src[lane + kWave * k];
+  for (int k = 0; k < VPL; ++k)
+    if (k < nk) dst[lane + kWave * k] = src[lane + kWave * k];
 }
 
 // Two-element bracket state held in scalars (a runtime-indexed local array
@@ -277,7 +282,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
   float tf = (float)t;
   if (c.mode) {
 #pragma unroll
-    for (int k = 0; k < VPL; ++k) {
+    for (int k = 0; k < VPL; ++k) if (k < c.nk) {
       int e = lane + kWave * k;
       float xv = x[e] + d[e] * tf;
       if (e < c.n3)
@@ -293,7 +298,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
     float tau[3] = {c.dv[3], c.dv[4], c.dv[5]}, jw[3];
     so3_jlt(w, tau, jw);
 #pragma unroll
-    for (int k = 0; k < VPL; ++k) {
+    for (int k = 0; k < VPL; ++k) if (k < c.nk) {
       int e = lane + kWave * k;
       g[e] = e < 3 ? c.dv[e]
                    : (e < 6 ? jw[e - 3]
@@ -304,7 +309,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
     return f;
   }
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) {
+  for (int k = 0; k < VPL; ++k) if (k < c.nk) {
     int e = lane + kWave * k;
     float xv = x[e] + d[e] * tf;
     if (e < c.n3)
@@ -315,7 +320,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
   __syncthreads();
   double f = eval_pose(*c.R, *c.L, c.G, c.li, c.s);
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) {
+  for (int k = 0; k < VPL; ++k) if (k < c.nk) {
     int e = lane + kWave * k;
     g[e] = e < c.n3 ? c.s.grad[e] : (e < c.n ? c.s.torsg[e - c.n3] : 0.f);
   }
@@ -356,14 +361,14 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
     *f_io = f_new;
     return;
   }
-  double gtd_new = vdot<VPL>(c.vec(b_new), d);
+  double gtd_new = vdot<VPL>(c.nk, c.vec(b_new), d);
   if (fabs(gtd_new) > p.gtd_threshold) {
     *f_io = f_new;
-    vcopy<VPL>(c.vec(V_G), c.vec(b_new));
+    vcopy<VPL>(c.nk, c.vec(V_G), c.vec(b_new));
     return;
   }
   double t_prev = 0.0, f_prev = f, gtd_prev = gtd;
-  vcopy<VPL>(c.vec(b_prev), c.vec(V_G));
+  vcopy<VPL>(c.nk, c.vec(b_prev), c.vec(V_G));
   bool done = false;
   int ls_iter = 0;
   while (ls_iter < p.max_ls) {
@@ -392,16 +397,16 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
     b_new = sw;
     gtd_prev = gtd_new;
     f_new = eval_dir(c, t, d, c.vec(b_new));
-    gtd_new = vdot<VPL>(c.vec(b_new), d);
+    gtd_new = vdot<VPL>(c.nk, c.vec(b_new), d);
     ls_iter++;
   }
   double d_norm = 0.0;
   if (ls_iter == p.max_ls) {
     br = {0.0, t}; bf = {f, f_new};
-    vcopy<VPL>(c.vec(b_prev), c.vec(V_G));  // bracket_g[0] = g
+    vcopy<VPL>(c.nk, c.vec(b_prev), c.vec(V_G));  // bracket_g[0] = g
     idx = {b_prev, b_new};
   } else {
-    d_norm = (double)vamax<VPL>(d);
+    d_norm = (double)vamax<VPL>(c.nk, d);
   }
   // a pool buffer outside the bracket receives each zoom evaluation
   int z_new = V_W0;
@@ -431,7 +436,7 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
       insuf = false;
     }
     f_new = eval_dir(c, t, d, c.vec(z_new));
-    gtd_new = vdot<VPL>(c.vec(z_new), d);
+    gtd_new = vdot<VPL>(c.nk, c.vec(z_new), d);
     ls_iter++;
     if (f_new > (f + p.c1 * t * gtd) || f_new >= bf.get(low)) {
       // bracket[high] <- new point; its old buffer becomes the zoom scratch
@@ -458,7 +463,7 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
   }
   *t_io = br.get(low);
   *f_io = bf.get(low);
-  vcopy<VPL>(c.vec(V_G), c.vec(idx.get(low)));
+  vcopy<VPL>(c.nk, c.vec(V_G), c.vec(idx.get(low)));
 }
 
 // lbfgs_step::apply (optimizer.cpp:429-560) on the pose held in the context:
@@ -471,21 +476,20 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
                             bool* converged_out) {
   const int lane = lane_id();
   const int m = P.history;
-  constexpr int NV = kWave * VPL;
+  const int NV = kWave * c.nk;  // history entries are packed at the vector length in use
   float* x = c.vec(V_X);
   float* g = c.vec(V_G);
   float* gp = c.vec(V_GP);
   float* d = c.vec(V_D);
   float* y = c.vec(V_W0);
   float* sv = c.vec(V_W1);
-  float* q = c.vec(V_W2);
 #pragma unroll
-  for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
+  for (int k = 0; k < VPL; ++k) if (k < c.nk) d[lane + kWave * k] = 0.f;
   // initial f(x), df/dx (optimizer.cpp:435-441)
   double loss = eval_dir(c, 0.0, d, g);
   double energy = loss;  // kept unless a step is taken (optimizer.cpp:554-558)
   bool converged = false, updated = false;
-  if (vamax<VPL>(g) < (float)P.tol_grad) {
+  if (vamax<VPL>(c.nk, g) < (float)P.tol_grad) {
     converged = true;
   } else {
     int head = 0, num_old = 0, n_iter = 0;
@@ -495,20 +499,20 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
       n_iter++;
       if (n_iter > 1) {
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) y[lane + kWave * k] = g[lane + kWave * k] - gp[lane + kWave * k];
-        high_change = vamax<VPL>(y) > 1e8f;
+        for (int k = 0; k < VPL; ++k) if (k < c.nk) y[lane + kWave * k] = g[lane + kWave * k] - gp[lane + kWave * k];
+        high_change = vamax<VPL>(c.nk, y) > 1e8f;
       }
       if (n_iter == 1 || high_change) {
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = -g[lane + kWave * k];
+        for (int k = 0; k < VPL; ++k) if (k < c.nk) d[lane + kWave * k] = -g[lane + kWave * k];
         num_old = 0;
         head = 0;
         H = 1.0;
       } else {
         float tf = (float)t;
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) sv[lane + kWave * k] = d[lane + kWave * k] * tf;
-        double ys = vdot<VPL>(y, sv);
+        for (int k = 0; k < VPL; ++k) if (k < c.nk) sv[lane + kWave * k] = d[lane + kWave * k] * tf;
+        double ys = vdot<VPL>(c.nk, y, sv);
         if (ys > 1e-10) {  // optimizer.cpp:473-487
           int slot;
           if (num_old == m) {
@@ -518,63 +522,119 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             slot = (head + num_old) % m;
             num_old++;
           }
-          H = ys / vdot<VPL>(y, y);
+          H = ys / vdot<VPL>(c.nk, y, y);
 #pragma unroll
-          for (int k = 0; k < VPL; ++k) {
+          for (int k = 0; k < VPL; ++k) if (k < c.nk) {
             hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
             hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
           }
           if (lane == 0) s_ro[slot] = 1.0 / ys;
           __syncthreads();
         }
-        // two-loop recursion (optimizer.cpp:493-507)
+        // two-loop recursion (optimizer.cpp:493-507).  q stays in registers;
+        // the next history entry is loaded while the current one is reduced,
+        // so the global-load latency leaves the dependent dot-product chain.
+        float qr[VPL], ns[VPL], ny[VPL];
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) q[lane + kWave * k] = -g[lane + kWave * k];
+        for (int k = 0; k < VPL; ++k) {
+          qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
+          ns[k] = 0.f;
+          ny[k] = 0.f;
+        }
+        if (num_old > 0) {
+          const size_t o = (size_t)((head + num_old - 1) % m) * NV + lane;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k)
+            if (k < c.nk) {
+              ns[k] = hs[o + kWave * k];
+              ny[k] = hy[o + kWave * k];
+            }
+        }
         for (int h = num_old - 1; h >= 0; --h) {
-          int slot = (head + h) % m;
-          const float* sh = hs + (size_t)slot * NV;
-          const float* yh = hy + (size_t)slot * NV;
+          const int slot = (head + h) % m;
+          float cs[VPL], cy[VPL];
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) {
+            cs[k] = ns[k];
+            cy[k] = ny[k];
+          }
+          if (h > 0) {
+            const size_t o = (size_t)((head + h - 1) % m) * NV + lane;
+#pragma unroll
+            for (int k = 0; k < VPL; ++k)
+              if (k < c.nk) {
+                ns[k] = hs[o + kWave * k];
+                ny[k] = hy[o + kWave * k];
+              }
+          }
           double acc = 0.0;
 #pragma unroll
           for (int k = 0; k < VPL; ++k)
-            acc += (double)sh[lane + kWave * k] * (double)q[lane + kWave * k];
+            if (k < c.nk) acc += (double)cs[k] * (double)qr[k];
           double al = wave_sum(acc) * s_ro[slot];
           if (lane == 0) s_al[h] = al;
           float alf = (float)al;
 #pragma unroll
-          for (int k = 0; k < VPL; ++k) q[lane + kWave * k] -= alf * yh[lane + kWave * k];
+          for (int k = 0; k < VPL; ++k)
+            if (k < c.nk) qr[k] -= alf * cy[k];
         }
         __syncthreads();
         float Hf = (float)H;
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = q[lane + kWave * k] * Hf;
+        for (int k = 0; k < VPL; ++k) qr[k] *= Hf;  // r = H q, kept in qr
+        if (num_old > 0) {
+          const size_t o = (size_t)(head % m) * NV + lane;
+#pragma unroll
+          for (int k = 0; k < VPL; ++k)
+            if (k < c.nk) {
+              ns[k] = hs[o + kWave * k];
+              ny[k] = hy[o + kWave * k];
+            }
+        }
         for (int h = 0; h < num_old; ++h) {
-          int slot = (head + h) % m;
-          const float* sh = hs + (size_t)slot * NV;
-          const float* yh = hy + (size_t)slot * NV;
+          const int slot = (head + h) % m;
+          float cs[VPL], cy[VPL];
+#pragma unroll
+          for (int k = 0; k < VPL; ++k) {
+            cs[k] = ns[k];
+            cy[k] = ny[k];
+          }
+          if (h + 1 < num_old) {
+            const size_t o = (size_t)((head + h + 1) % m) * NV + lane;
+#pragma unroll
+            for (int k = 0; k < VPL; ++k)
+              if (k < c.nk) {
+                ns[k] = hs[o + kWave * k];
+                ny[k] = hy[o + kWave * k];
+              }
+          }
           double acc = 0.0;
 #pragma unroll
           for (int k = 0; k < VPL; ++k)
-            acc += (double)yh[lane + kWave * k] * (double)d[lane + kWave * k];
+            if (k < c.nk) acc += (double)cy[k] * (double)qr[k];
           double be = wave_sum(acc) * s_ro[slot];
           float cf = (float)(s_al[h] - be);
 #pragma unroll
-          for (int k = 0; k < VPL; ++k) d[lane + kWave * k] += sh[lane + kWave * k] * cf;
+          for (int k = 0; k < VPL; ++k)
+            if (k < c.nk) qr[k] += cs[k] * cf;
         }
+#pragma unroll
+        for (int k = 0; k < VPL; ++k)
+          if (k < c.nk) d[lane + kWave * k] = qr[k];
       }
-      vcopy<VPL>(gp, g);
+      vcopy<VPL>(c.nk, gp, g);
       prev_loss = loss;
       if (n_iter == 1 || high_change) {
         double l1 = 0.0;
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) l1 += fabs((double)g[lane + kWave * k]);
+        for (int k = 0; k < VPL; ++k) if (k < c.nk) l1 += fabs((double)g[lane + kWave * k]);
         l1 = wave_sum(l1);
         double inv = 1.0 / l1;
         t = (1.0 < inv ? 1.0 : inv) * P.lr * 0.1;
       } else {
         t = P.lr;
       }
-      double gtd = vdot<VPL>(g, d);
+      double gtd = vdot<VPL>(c.nk, g, d);
       if (gtd > -P.tol_change) {
         converged = true;
         break;
@@ -584,12 +644,12 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
       float tf = (float)t;
       float smax = 0.f;
 #pragma unroll
-      for (int k = 0; k < VPL; ++k) {
+      for (int k = 0; k < VPL; ++k) if (k < c.nk) {
         x[lane + kWave * k] += d[lane + kWave * k] * tf;
         smax = fmaxf(smax, fabsf(d[lane + kWave * k] * tf));
       }
       updated = true;
-      if (vamax<VPL>(g) <= (float)P.tol_grad) {
+      if (vamax<VPL>(c.nk, g) <= (float)P.tol_grad) {
         converged = true;
         break;
       }
@@ -605,7 +665,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
     // optimizer.cpp:553-558: one final evaluation only after a step
     if (updated) {
 #pragma unroll
-      for (int k = 0; k < VPL; ++k) d[lane + kWave * k] = 0.f;
+      for (int k = 0; k < VPL; ++k) if (k < c.nk) d[lane + kWave * k] = 0.f;
       energy = eval_dir(c, 0.0, d, g);
     }
   }
@@ -615,7 +675,8 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 
 template <int VPL, int MODE>
 __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
-                                              const DevGrid* __restrict__ G, LbfgsArgs A) {
+                                              const DevGrid* __restrict__ G, LbfgsArgs A,
+                                              int vpl_lo) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
@@ -647,9 +708,11 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
     __syncthreads();
     if (pi >= A.n_poses) break;
     c.li = L.info[A.pose_lig[pi]];
-    load_ligand_params(L, c.li, G, c.s);
     c.n3 = MODE ? 6 + c.li.n_tors : 3 * c.li.n_atoms;
     c.n = c.n3 + R.n_leaves;
+    c.nk = (c.n + kWave - 1) / kWave;
+    if (c.nk <= vpl_lo || c.nk > VPL) continue;  // another size class runs this pose
+    load_ligand_params(L, c.li, G, c.s);
     c.nevals = 0;
     const long long off = 3ll * A.pose_off[pi];
     if (MODE) {
@@ -698,10 +761,21 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
 
 void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st) {
-  if (a.prm.dof_mode)
-    hipLaunchKernelGGL((k_lbfgs<kLbfgsVpl, 1>), dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
-  else
-    hipLaunchKernelGGL((k_lbfgs<kLbfgsVpl, 0>), dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+  // rigid-body + torsion variables: n <= 6 + 31 + 64 always fits width 2
+  if (a.prm.dof_mode) {
+    hipLaunchKernelGGL((k_lbfgs<2, 1>), dim3(n_slots), dim3(kWave), 0, st, R, L, G, a, 0);
+    return;
+  }
+  static_assert(kLbfgsClasses == 3, "one launch per class below");
+  LbfgsArgs c = a;
+  hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[2], 0>), dim3(n_slots), dim3(kWave), 0, st, R, L,
+                     G, c, kLbfgsClassVpl[1]);
+  c.counter = a.counter + 1;
+  hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[1], 0>), dim3(n_slots), dim3(kWave), 0, st, R, L,
+                     G, c, kLbfgsClassVpl[0]);
+  c.counter = a.counter + 2;
+  hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[0], 0>), dim3(n_slots), dim3(kWave), 0, st, R, L,
+                     G, c, 0);
 }
 
 // ============================================================== K5: pscore
@@ -862,6 +936,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
     const int n = li.n_atoms, n3 = 3 * n;
     c.n3 = n3;
     c.n = n3 + nl;
+    c.nk = (c.n + kWave - 1) / kWave;
     load_ligand_params(L, li, G, c.s);
     Rng rng{P.seed_lo, P.seed_hi + (uint32_t)pid, (uint32_t)A.walker_lig[w], st.rng_ctr};
     const long long off = 3ll * A.walker_off[w];

@@ -50,9 +50,15 @@ struct LbfgsArgs {
   double* energy;
   int* nevals;
   int* converged;
-  int* counter;   // work-queue head, zero before launch
+  int* counter;   // work-queue heads (one per size class), zero before launch
   float* hist;    // [n_slots][2][history][64*kLbfgsVpl]
 };
+
+// L-BFGS size classes: a launch per class takes the poses whose variable
+// count n satisfies 64 * (lower class width) < n <= 64 * width, so its LDS
+// vector pool (and its occupancy) follows the poses it runs.
+constexpr int kLbfgsClasses = 3;
+constexpr int kLbfgsClassVpl[kLbfgsClasses] = {2, 4, kLbfgsVpl};
 
 struct PscoreArgs {
   int n_poses;
