@@ -1170,7 +1170,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
     spc = g_tune.lbfgs_slots_per_cu;
   }
   int n_slots = std::min(n_poses, std::max(1, ncu) * spc);
-  size_t per_slot = (size_t)2 * prm->history * kWave * kLbfgsVpl;
+  size_t per_slot = lbfgs_slot_floats(prm->history);
   std::lock_guard<std::mutex> lock(g_ws.mu);
   if (g_ws.hist_floats < per_slot * n_slots) {
     if (g_ws.hist) HIP_TRY(hipFree(g_ws.hist));

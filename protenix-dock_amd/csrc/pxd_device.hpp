@@ -21,8 +21,12 @@ namespace pxd {
 // which is read back as a uniform value.  GFX9 DPP controls: quad_perm 0xB1
 // (xor 1) and 0x4E (xor 2), row_half_mirror 0x141, row_mirror 0x140,
 // row_bcast:15 0x142 (rows 1 and 3), row_bcast:31 0x143 (rows 2 and 3).
+// With all rows enabled every lane is written and the old value is unused
+// (mov_dpp, no initialising move); with rows masked off the disabled rows
+// keep old = 0 (update_dpp), so the following add leaves them unchanged.
 template <int CTRL, int ROWS>
 __device__ inline float dpp_f(float v) {
+  if (ROWS == 0xF) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
   return __int_as_float(
       __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
@@ -30,8 +34,14 @@ __device__ inline float dpp_f(float v) {
 template <int CTRL, int ROWS>
 __device__ inline double dpp_d(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  int lo, hi;
+  if (ROWS == 0xF) {
+    lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, ROWS, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, false);
+  }
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 

@@ -532,91 +532,70 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
           __syncthreads();
         }
         // two-loop recursion (optimizer.cpp:493-507).  q stays in registers;
-        // the next history entry is loaded while the current one is reduced,
-        // so the global-load latency leaves the dependent dot-product chain.
-        float qr[VPL], ns[VPL], ny[VPL];
+        // history entries are loaded kPre ahead of their use (a ring of
+        // register buffers, statically indexed by unrolling kPre steps), so
+        // the global-load latency leaves the dependent dot-product chain.
+        // The arithmetic does not depend on kPre or VPL: every size class
+        // gives the same result for the same pose.
+        constexpr int kPre = VPL <= 2 ? 6 : (VPL <= 4 ? 4 : 2);
+        float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
 #pragma unroll
-        for (int k = 0; k < VPL; ++k) {
-          qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
-          ns[k] = 0.f;
-          ny[k] = 0.f;
-        }
-        if (num_old > 0) {
-          const size_t o = (size_t)((head + num_old - 1) % m) * NV + lane;
+        for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
+        auto fetch = [&](int h, float* ds, float* dy) {
+          const size_t o = (size_t)((head + h) % m) * NV + lane;
 #pragma unroll
           for (int k = 0; k < VPL; ++k)
             if (k < c.nk) {
-              ns[k] = hs[o + kWave * k];
-              ny[k] = hy[o + kWave * k];
+              ds[k] = hs[o + kWave * k];
+              dy[k] = hy[o + kWave * k];
             }
-        }
-        for (int h = num_old - 1; h >= 0; --h) {
-          const int slot = (head + h) % m;
-          float cs[VPL], cy[VPL];
+        };
 #pragma unroll
-          for (int k = 0; k < VPL; ++k) {
-            cs[k] = ns[k];
-            cy[k] = ny[k];
+        for (int j = 0; j < kPre; ++j)
+          if (num_old - 1 - j >= 0) fetch(num_old - 1 - j, bs[j], by[j]);
+        for (int h0 = num_old - 1; h0 >= 0; h0 -= kPre) {
+#pragma unroll
+          for (int j = 0; j < kPre; ++j) {
+            const int h = h0 - j;
+            if (h >= 0) {
+              double acc = 0.0;
+#pragma unroll
+              for (int k = 0; k < VPL; ++k)
+                if (k < c.nk) acc += (double)bs[j][k] * (double)qr[k];
+              double al = wave_sum(acc) * s_ro[(head + h) % m];
+              if (lane == 0) s_al[h] = al;
+              float alf = (float)al;
+#pragma unroll
+              for (int k = 0; k < VPL; ++k)
+                if (k < c.nk) qr[k] -= alf * by[j][k];
+              if (h - kPre >= 0) fetch(h - kPre, bs[j], by[j]);
+            }
           }
-          if (h > 0) {
-            const size_t o = (size_t)((head + h - 1) % m) * NV + lane;
-#pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              if (k < c.nk) {
-                ns[k] = hs[o + kWave * k];
-                ny[k] = hy[o + kWave * k];
-              }
-          }
-          double acc = 0.0;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) acc += (double)cs[k] * (double)qr[k];
-          double al = wave_sum(acc) * s_ro[slot];
-          if (lane == 0) s_al[h] = al;
-          float alf = (float)al;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) qr[k] -= alf * cy[k];
         }
         __syncthreads();
         float Hf = (float)H;
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] *= Hf;  // r = H q, kept in qr
-        if (num_old > 0) {
-          const size_t o = (size_t)(head % m) * NV + lane;
 #pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) {
-              ns[k] = hs[o + kWave * k];
-              ny[k] = hy[o + kWave * k];
+        for (int j = 0; j < kPre; ++j)
+          if (j < num_old) fetch(j, bs[j], by[j]);
+        for (int h0 = 0; h0 < num_old; h0 += kPre) {
+#pragma unroll
+          for (int j = 0; j < kPre; ++j) {
+            const int h = h0 + j;
+            if (h < num_old) {
+              double acc = 0.0;
+#pragma unroll
+              for (int k = 0; k < VPL; ++k)
+                if (k < c.nk) acc += (double)by[j][k] * (double)qr[k];
+              double be = wave_sum(acc) * s_ro[(head + h) % m];
+              float cf = (float)(s_al[h] - be);
+#pragma unroll
+              for (int k = 0; k < VPL; ++k)
+                if (k < c.nk) qr[k] += bs[j][k] * cf;
+              if (h + kPre < num_old) fetch(h + kPre, bs[j], by[j]);
             }
-        }
-        for (int h = 0; h < num_old; ++h) {
-          const int slot = (head + h) % m;
-          float cs[VPL], cy[VPL];
-#pragma unroll
-          for (int k = 0; k < VPL; ++k) {
-            cs[k] = ns[k];
-            cy[k] = ny[k];
           }
-          if (h + 1 < num_old) {
-            const size_t o = (size_t)((head + h + 1) % m) * NV + lane;
-#pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              if (k < c.nk) {
-                ns[k] = hs[o + kWave * k];
-                ny[k] = hy[o + kWave * k];
-              }
-          }
-          double acc = 0.0;
-#pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) acc += (double)cy[k] * (double)qr[k];
-          double be = wave_sum(acc) * s_ro[slot];
-          float cf = (float)(s_al[h] - be);
-#pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) qr[k] += cs[k] * cf;
         }
 #pragma unroll
         for (int k = 0; k < VPL; ++k)
@@ -685,7 +664,7 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   __shared__ int s_pose;
   const int lane = lane_id();
   constexpr int NV = kWave * VPL;
-  float* hy = A.hist + (size_t)blockIdx.x * 2 * A.prm.history * NV;
+  float* hy = A.hist + (size_t)blockIdx.x * lbfgs_slot_floats(A.prm.history);
   float* hs = hy + (size_t)A.prm.history * NV;
   LbfgsCtx<VPL> c;
   c.R = &R;
@@ -884,7 +863,7 @@ enum { PH_INIT = 0, PH_MUTATE = 1, PH_REFINE = 2 };
 
 template <int VPL>
 __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
-                                           const DevGrid* __restrict__ G, McArgs A) {
+                                           const DevGrid* __restrict__ G, McArgs A, int vpl_lo) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
@@ -899,7 +878,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   const McParamsDev& P = A.prm;
   const pxd_score_params& sp = *A.sp;
   const int nl = R.n_leaves;
-  float* hy = A.hist + (size_t)blockIdx.x * 2 * A.hist_m * NV;
+  float* hy = A.hist + (size_t)blockIdx.x * lbfgs_slot_floats(A.hist_m);
   float* hs = hy + (size_t)A.hist_m * NV;
   LbfgsCtx<VPL> c;
   c.R = &R;
@@ -928,15 +907,16 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
     const int w = s_w;
     __syncthreads();
     if (w >= A.n_walkers) break;
-    WalkerState st = A.st[w];
-    if (st.mode == 2) continue;
-    const int pid = A.walker_pid[w];
     c.li = L.info[A.walker_lig[w]];
     const LigInfo& li = c.li;
     const int n = li.n_atoms, n3 = 3 * n;
     c.n3 = n3;
     c.n = n3 + nl;
     c.nk = (c.n + kWave - 1) / kWave;
+    if (c.nk <= vpl_lo || c.nk > VPL) continue;  // another size class runs this walker
+    WalkerState st = A.st[w];
+    if (st.mode == 2) continue;
+    const int pid = A.walker_pid[w];
     load_ligand_params(L, li, G, c.s);
     Rng rng{P.seed_lo, P.seed_hi + (uint32_t)pid, (uint32_t)A.walker_lig[w], st.rng_ctr};
     const long long off = 3ll * A.walker_off[w];
@@ -1124,7 +1104,16 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
 
 void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
                int n_slots, hipStream_t st) {
-  hipLaunchKernelGGL(k_mc<kLbfgsVpl>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, a);
+  // one launch per L-BFGS size class (the walker's ligand decides), each
+  // with its own queue head: a.counter[0..kLbfgsClasses)
+  McArgs c = a;
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[2]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c,
+                     kLbfgsClassVpl[1]);
+  c.counter = a.counter + 1;
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[1]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c,
+                     kLbfgsClassVpl[0]);
+  c.counter = a.counter + 2;
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[0]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c, 0);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {

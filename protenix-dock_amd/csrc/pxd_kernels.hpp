@@ -51,8 +51,13 @@ struct LbfgsArgs {
   int* nevals;
   int* converged;
   int* counter;   // work-queue heads (one per size class), zero before launch
-  float* hist;    // [n_slots][2][history][64*kLbfgsVpl]
+  float* hist;    // [n_slots][lbfgs_slot_floats(history)]
 };
+
+// Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each.
+__host__ __device__ inline size_t lbfgs_slot_floats(int history) {
+  return (size_t)2 * history * kWave * kLbfgsVpl;
+}
 
 // L-BFGS size classes: a launch per class takes the poses whose variable
 // count n satisfies 64 * (lower class width) < n <= 64 * width, so its LDS
@@ -102,7 +107,7 @@ struct McArgs {
   float* cur_tors;
   float* best_tors;
   WalkerState* st;
-  int* counter;
+  int* counter;    // queue heads, one per L-BFGS size class
   float* hist;
   int hist_m;      // history slots per wave in hist
   int step_end;

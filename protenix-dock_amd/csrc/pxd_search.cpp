@@ -111,7 +111,7 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   const int n_slots = std::min(W, std::max(1, ncu) * 4);
   const int hist_m = std::max(prm->relax.history, prm->full.history);
   const int n_launch = (prm->max_nsteps + prm->steps_per_launch - 1) / prm->steps_per_launch + 1;
-  const size_t hist_floats = (size_t)n_slots * 2 * hist_m * kWave * kLbfgsVpl;
+  const size_t hist_floats = (size_t)n_slots * lbfgs_slot_floats(hist_m);
   hipError_t err = hipSuccess;
   hipStream_t user = static_cast<hipStream_t>(stream), cs = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -130,7 +130,7 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   };
   const size_t o_sp = take(sizeof(pxd_score_params)), o_wlig = take(4 * W), o_wpid = take(4 * W),
                o_woff = take(4 * (W + 1)), o_lwoff = take(4 * (nl + 1)),
-               o_cnt = take(4 * (size_t)n_launch), o_cur = take(12 * (size_t)atoms),
+               o_cnt = take(4 * (size_t)n_launch * kLbfgsClasses), o_cur = take(12 * (size_t)atoms),
                o_curt = take(4 * (size_t)W * (L + 1)), o_st = take(sizeof(WalkerState) * W),
                o_hist = take(4 * hist_floats);
   HIP_OK(hipMalloc(&mem, off));
@@ -189,9 +189,9 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
     // then one k_mc launch per segment of steps, then clustering
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
-    launch_fill_int(d_cnt, n_launch, 0, cs);
+    launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, cs);
     for (int k = 0; k < n_launch; ++k) {
-      a.counter = d_cnt + k;
+      a.counter = d_cnt + k * kLbfgsClasses;
       a.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
       launch_mc(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, a, n_slots, cs);
     }
