@@ -175,6 +175,74 @@ def search_leg(args, rec, box, R, G, workers):
     return d
 
 
+def _make_large_ligand(i):
+    from pxdock_amd import synth
+    rng = np.random.default_rng(50000 + i + 7919)
+    nh = int(rng.integers(10, 61))
+    nt = int(rng.integers(0, 21))
+    nr = int(rng.integers(0, 4))
+    return synth.make_ligand(50000 + i, n_heavy=nh, n_torsions=nt, n_rings=nr)
+
+
+def _make_large_poses(args):
+    i, lig, lo, hi, n = args
+    return _make_poses((100000 + i, lig, lo, hi, n))
+
+
+def large_leg(args, workers):
+    """Large workload: a synthetic 8,000-heavy-atom receptor (same recipe as the
+    2,000-atom one, sphere radius scaled to keep the density), a 40 A box at
+    0.375 A (109^3 points), the maps of the screening ligands' types built on
+    the GPU; then L-BFGS to convergence (reference full step) on a sample of
+    the screening batch: args.large_ligands ligands (10-60 heavy atoms, 0-20
+    torsions) x 16 random starts in the box."""
+    import torch
+    from concurrent.futures import ProcessPoolExecutor
+
+    from pxdock_amd import ffi, lib, synth
+    t0 = time.time()
+    rec = synth.make_receptor(2, n_heavy=8000, radius=47.6, pocket_offset=24.0, n_ring6=80,
+                              n_ring5=24)
+    box = lib.decide_grid_dims(rec.pocket_center, [40.0] * 3, 0.375)
+    lo, hi = np.array(box.init_xyz), np.array(box.oppo_xyz)
+    per = 16
+    with ProcessPoolExecutor(workers) as ex:
+        ligs = list(ex.map(_make_large_ligand, range(args.large_ligands), chunksize=16))
+        poses = list(ex.map(_make_large_poses,
+                            [(i, l, lo, hi, per) for i, l in enumerate(ligs)], chunksize=16))
+    log(f"large workload generated in {time.time() - t0:.1f}s")
+    types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
+    R = lib.Receptor(rec)
+    lib.Grid(R, box, types[:1])  # first launch of the build kernels
+    torch.cuda.synchronize()
+    tg = time.time()
+    G = lib.Grid(R, box, types)
+    torch.cuda.synchronize()
+    grid_s = time.time() - tg
+    grid_bytes = (len(types) + 1) * box.n_points() * 16
+    B = lib.LigandBatch(ligs)
+    pose_lig = np.repeat(np.arange(len(ligs)), per)
+    P = lib.PoseSet(B, pose_lig, [p for arr in poses for p in arr], rec.leaves.n_leaves)
+    prm = ffi.lbfgs_params(240)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    _, ne, cv = lib.lbfgs(R, G, P, prm)
+    torch.cuda.synchronize()
+    dt = time.time() - t1
+    cv, ne = cv.cpu().numpy(), ne.cpu().numpy()
+    d = {"receptor_heavy_atoms": 8000, "receptor_atoms": int(rec.n_atoms), "box_A": 40.0,
+         "grid_points": box.npts, "maps": len(types) + 1,
+         "grid_build": {"ms": grid_s * 1e3, "output_GB_per_s": grid_bytes / grid_s / 1e9,
+                        "output_bytes": grid_bytes},
+         "lbfgs": {"ligands": len(ligs), "starts_per_ligand": per, "poses": int(P.n_poses),
+                   "seconds": dt, "converged_frac": float(cv.mean()),
+                   "converged_poses_per_s": float(cv.sum() / dt),
+                   "evals_per_s": float(ne.sum() / dt)},
+         "sample": f"{len(ligs)} of the 65,536 screening ligands x {per} starts"}
+    log(f"large: {d}")
+    return d
+
+
 def _make_train_ligand(i):
     from pxdock_amd import synth
     rng = np.random.default_rng(40000 + i + 7919)
@@ -279,10 +347,15 @@ def main():
     ap.add_argument("--no-search", action="store_true")
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-ligands", type=int, default=4096)
     ap.add_argument("--train-poses", type=int, default=16384)
     ap.add_argument("--search-ligands", type=int, default=256)
     ap.add_argument("--search-walkers", type=int, default=64)
     ap.add_argument("--search-steps", type=int, default=10)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device three times (a known-size "
+                         "kernel for calibrating the HBM traffic counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: sharding the path across GPUs (by ligand) "
@@ -290,6 +363,15 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_copy": a.numel() * 4, "copies": 3}), flush=True)
+        return
 
     from pxdock_amd import ffi, lib
     workers = min(os.cpu_count() or 1, 16)
@@ -463,6 +545,8 @@ def main():
         out["search"] = search_leg(args, rec, box, R, G, workers)
     if not args.no_train:
         out["train"] = train_leg(args, rec, R, workers)
+    if not args.no_large:
+        out["large"] = large_leg(args, workers)
     print(json.dumps(out), flush=True)
 
 
