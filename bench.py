@@ -95,6 +95,14 @@ def flops_bytes_per_pose(lig, n_excl):
     return f, b
 
 
+def grid_flops(rec, n_points, n_types):
+    """Algorithmic FLOPs of a map build: every grid point against every
+    receptor atom outside the rotatable leaves (grid.cpp:164-192), 26 FLOPs
+    for the Coulomb probe and 30 per LJ type map."""
+    n_fixed = rec.n_atoms - len(rec.leaves.movable())
+    return float(n_points) * n_fixed * (26.0 + 30.0 * n_types)
+
+
 def _make_search_ligand(i):
     from pxdock_amd import synth
     rng = np.random.default_rng(30000 + i + 7919)
@@ -233,7 +241,8 @@ def large_leg(args, workers):
     d = {"receptor_heavy_atoms": 8000, "receptor_atoms": int(rec.n_atoms), "box_A": 40.0,
          "grid_points": box.npts, "maps": len(types) + 1,
          "grid_build": {"ms": grid_s * 1e3, "output_GB_per_s": grid_bytes / grid_s / 1e9,
-                        "output_bytes": grid_bytes},
+                        "output_bytes": grid_bytes,
+                        "tflops": grid_flops(rec, box.n_points(), len(types)) / grid_s / 1e12},
          "lbfgs": {"ligands": len(ligs), "starts_per_ligand": per, "poses": int(P.n_poses),
                    "seconds": dt, "converged_frac": float(cv.mean()),
                    "converged_poses_per_s": float(cv.sum() / dt),
@@ -455,7 +464,8 @@ def main():
         "roofline": roof,
         "tuning": tune,
         "grid_build": {"ms": grid_ms, "output_GB_per_s": grid_bytes / (grid_ms / 1e3) / 1e9,
-                       "maps": len(types) + 1, "points": box.n_points()},
+                       "maps": len(types) + 1, "points": box.n_points(),
+                       "tflops": grid_flops(rec, box.n_points(), len(types)) / grid_ms / 1e9},
     }
 
     # ---------------- L-BFGS: persistent batched vs plain one-pose-per-launch

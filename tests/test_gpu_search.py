@@ -197,3 +197,53 @@ def test_gpu_apply_poses_matches_oracle(sw):
         ref = O.ligand_apply(lig.tree, xin[p], cen[p], rot[p].reshape(3, 3).ravel(), tors[p])
         got = out[off[p]:off[p] + lig.n_atoms]
         assert np.abs(got - ref).max() <= 1e-4, (p, np.abs(got - ref).max())
+
+
+def test_gpu_search_distributions_over_seeds_match_oracle(sw):
+    """With the optimiser on, FP32 and FP64 walks part ways after a few steps,
+    so the search is compared as a stochastic process: over 6 seeds x 4
+    walkers per ligand, the best pscores found and the RMSDs of the walkers'
+    best poses to the ligand's best pose must come from the same
+    distributions on the GPU and in the oracle (two-sample Kolmogorov-Smirnov
+    test not rejecting at 0.1 %), and the best pscore found over all seeds
+    must agree within 0.5 kcal/mol."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from scipy.stats import ks_2samp
+    seeds, walkers, steps = range(1, 7), 4, 8
+    gpu = {l: [] for l in range(len(sw["ligs"]))}
+    jobs = []
+    for seed in seeds:
+        prm = ffi.mc_params(seed=seed, max_nsteps=steps, relax_iters=10, full_iters=30)
+        print(f"\n[mc] distribution seed={seed}", flush=True)
+        res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"],
+                            walkers)
+        bp = res.best_pscore.cpu().numpy()
+        for l in range(len(sw["ligs"])):
+            for k in range(walkers):
+                w = l * walkers + k
+                gpu[l].append((float(bp[w]), res.walker_xyz(w)))
+                jobs.append((l, k, prm))
+    with ThreadPoolExecutor(8) as ex:
+        outs = list(ex.map(lambda j: (j[0], O.mc_walker(sw["systems"][j[0]], sw["ligs"][j[0]],
+                                                        sw["sp"], j[2], sw["box"], j[1], j[0],
+                                                        sw["x0"][j[0]])), jobs))
+    ora = {l: [] for l in gpu}
+    for l, o in outs:
+        ora[l].append((o["pscore"], o["xyz"]))
+
+    def rmsd_to_best(rows):
+        best = min(rows, key=lambda r: r[0])[1]
+        return [float(np.sqrt(((r[1] - best) ** 2).sum(1).mean())) for r in rows]
+
+    for l in gpu:
+        gs = np.array([r[0] for r in gpu[l]])
+        os_ = np.array([r[0] for r in ora[l]])
+        gr, orr = rmsd_to_best(gpu[l]), rmsd_to_best(ora[l])
+        p_s = ks_2samp(gs, os_).pvalue
+        p_r = ks_2samp(gr, orr).pvalue
+        print(f"   ligand {l}: best pscore gpu {gs.min():.3f} oracle {os_.min():.3f}; "
+              f"median gpu {np.median(gs):.3f} oracle {np.median(os_):.3f}; "
+              f"KS p(score) {p_s:.3g} p(rmsd) {p_r:.3g}")
+        assert p_s > 1e-3 and p_r > 1e-3
+        assert abs(gs.min() - os_.min()) <= 0.5
