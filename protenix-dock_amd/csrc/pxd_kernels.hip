@@ -532,44 +532,48 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
           __syncthreads();
         }
         // two-loop recursion (optimizer.cpp:493-507).  q stays in registers;
-        // history entries are loaded kPre ahead of their use (a ring of
-        // register buffers, statically indexed by unrolling kPre steps), so
-        // the global-load latency leaves the dependent dot-product chain.
+        // history entries are loaded kPre steps ahead of their use into a
+        // ring of register buffers (statically indexed by unrolling kPre
+        // steps).  Every load is unconditional — indices are clamped, and a
+        // step past the last entry or an element past n is masked out of the
+        // arithmetic instead of branched around — so the compiler can count
+        // the loads in flight and only waits for the one entry it needs.
         // The arithmetic does not depend on kPre or VPL: every size class
         // gives the same result for the same pose.
         constexpr int kPre = VPL <= 2 ? 6 : (VPL <= 4 ? 4 : 2);
+        const int nk1 = c.nk - 1;
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
-        auto fetch = [&](int h, float* ds, float* dy) {
-          const size_t o = (size_t)((head + h) % m) * NV + lane;
+        auto fetch = [&](int h, float* ds, float* dy) {  // h clamped to [0, num_old)
+          h = min(max(h, 0), max(num_old - 1, 0));
+          const size_t o = (size_t)(((unsigned)head + (unsigned)h) % (unsigned)m) * NV + lane;
 #pragma unroll
-          for (int k = 0; k < VPL; ++k)
-            if (k < c.nk) {
-              ds[k] = hs[o + kWave * k];
-              dy[k] = hy[o + kWave * k];
-            }
+          for (int k = 0; k < VPL; ++k) {
+            const int kk = min(k, nk1);
+            ds[k] = hs[o + kWave * kk];
+            dy[k] = hy[o + kWave * kk];
+          }
         };
+        const int ngrp = (num_old + kPre - 1) / kPre;
+        const unsigned um = (unsigned)m, uhead = (unsigned)head;
 #pragma unroll
-        for (int j = 0; j < kPre; ++j)
-          if (num_old - 1 - j >= 0) fetch(num_old - 1 - j, bs[j], by[j]);
-        for (int h0 = num_old - 1; h0 >= 0; h0 -= kPre) {
+        for (int j = 0; j < kPre; ++j) fetch(num_old - 1 - j, bs[j], by[j]);
+        for (int grp = 0; grp < ngrp; ++grp) {
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
-            const int h = h0 - j;
-            if (h >= 0) {
-              double acc = 0.0;
+            const int h = num_old - 1 - (grp * kPre + j);
+            const double sro = s_ro[(uhead + (unsigned)max(h, 0)) % um];
+            double acc = 0.0;
 #pragma unroll
-              for (int k = 0; k < VPL; ++k)
-                if (k < c.nk) acc += (double)bs[j][k] * (double)qr[k];
-              double al = wave_sum(acc) * s_ro[(head + h) % m];
-              if (lane == 0) s_al[h] = al;
-              float alf = (float)al;
+            for (int k = 0; k < VPL; ++k)
+              acc += k <= nk1 ? (double)bs[j][k] * (double)qr[k] : 0.0;
+            const double al = wave_sum(acc) * sro * (h >= 0 ? 1.0 : 0.0);
+            s_al[h >= 0 ? h : kMaxHistory] = al;  // every lane stores the same value
+            const float alf = (float)al;
 #pragma unroll
-              for (int k = 0; k < VPL; ++k)
-                if (k < c.nk) qr[k] -= alf * by[j][k];
-              if (h - kPre >= 0) fetch(h - kPre, bs[j], by[j]);
-            }
+            for (int k = 0; k < VPL; ++k) qr[k] -= (k <= nk1 ? alf : 0.f) * by[j][k];
+            fetch(h - kPre, bs[j], by[j]);
           }
         }
         __syncthreads();
@@ -577,24 +581,23 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] *= Hf;  // r = H q, kept in qr
 #pragma unroll
-        for (int j = 0; j < kPre; ++j)
-          if (j < num_old) fetch(j, bs[j], by[j]);
-        for (int h0 = 0; h0 < num_old; h0 += kPre) {
+        for (int j = 0; j < kPre; ++j) fetch(j, bs[j], by[j]);
+        for (int grp = 0; grp < ngrp; ++grp) {
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
-            const int h = h0 + j;
-            if (h < num_old) {
-              double acc = 0.0;
+            const int h = grp * kPre + j;
+            const int hc = min(h, num_old - 1);  // a step past the newest entry reads a valid one
+            const double sro = s_ro[(uhead + (unsigned)hc) % um];
+            const double sal = s_al[hc];
+            double acc = 0.0;
 #pragma unroll
-              for (int k = 0; k < VPL; ++k)
-                if (k < c.nk) acc += (double)by[j][k] * (double)qr[k];
-              double be = wave_sum(acc) * s_ro[(head + h) % m];
-              float cf = (float)(s_al[h] - be);
+            for (int k = 0; k < VPL; ++k)
+              acc += k <= nk1 ? (double)by[j][k] * (double)qr[k] : 0.0;
+            const double be = wave_sum(acc) * sro;
+            const float cf = (float)(sal - be) * (h < num_old ? 1.f : 0.f);
 #pragma unroll
-              for (int k = 0; k < VPL; ++k)
-                if (k < c.nk) qr[k] += bs[j][k] * cf;
-              if (h + kPre < num_old) fetch(h + kPre, bs[j], by[j]);
-            }
+            for (int k = 0; k < VPL; ++k) qr[k] += bs[j][k] * (k <= nk1 ? cf : 0.f);
+            fetch(h + kPre, bs[j], by[j]);
           }
         }
 #pragma unroll
@@ -659,7 +662,7 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
-  __shared__ double s_al[kMaxHistory];
+  __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ __align__(16) float dofl[MODE ? PXD_DOF_LDS_FLOATS : 1];
   __shared__ int s_pose;
   const int lane = lane_id();
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_ro[kMaxHistory];
-  __shared__ double s_al[kMaxHistory];
+  __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ double s_terms[PXD_NUM_TERMS];
   __shared__ float apl[PXD_APPLY_LDS_FLOATS];
