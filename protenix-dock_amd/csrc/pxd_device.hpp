@@ -705,44 +705,48 @@ __device__ inline double ligand_pairs_masked(const DevLigands& L, const LigInfo&
     q1 = s.q[a1];
     l1 = s.lj[a1];
   }
+  // partners in blocks of 32 (one mask word per block, read once); the pair
+  // energies of a block are summed in FP32 and added to the FP64 total once
   double e = 0.0;
-  for (int b = 0; b < n; ++b) {
-    const float xb = s.xyz[3 * b], yb = s.xyz[3 * b + 1], zb = s.xyz[3 * b + 2], qb = s.q[b];
-    const float2 lb = s.lj[b];
-    const int w = b >> 5;
-    const uint32_t bit = 1u << (b & 31);
-    {
-      uint32_t wa = w == 0 ? m0a.x : w == 1 ? m0a.y : w == 2 ? m0a.z : m0a.w;
-      uint32_t wf = w == 0 ? m0f.x : w == 1 ? m0f.y : w == 2 ? m0f.z : m0f.w;
-      bool ia = wa & bit, i4 = wf & bit;
-      if (ia || i4) {
-        float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
-        float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
-        float rx = xb - x0[0], ry = yb - x0[1], rz = zb - x0[2], de;
-        float ep = pair_energy(rx, ry, rz, q0 * qb * cs, (l0.x + lb.x) * 0.5f,
-                               l0.y * lb.y * 4.f * vs, &de);
-        e += 0.5 * (double)ep;
-        g0[0] -= rx * de;
-        g0[1] -= ry * de;
-        g0[2] -= rz * de;
+  for (int w = 0; 32 * w < n; ++w) {
+    const uint32_t wa0 = w == 0 ? m0a.x : w == 1 ? m0a.y : w == 2 ? m0a.z : m0a.w;
+    const uint32_t wf0 = w == 0 ? m0f.x : w == 1 ? m0f.y : w == 2 ? m0f.z : m0f.w;
+    const uint32_t wa1 = w == 0 ? m1a.x : w == 1 ? m1a.y : w == 2 ? m1a.z : m1a.w;
+    const uint32_t wf1 = w == 0 ? m1f.x : w == 1 ? m1f.y : w == 2 ? m1f.z : m1f.w;
+    const int bend = min(32, n - 32 * w);
+    float es = 0.f;
+    for (int t = 0; t < bend; ++t) {
+      const int b = 32 * w + t;
+      const float xb = s.xyz[3 * b], yb = s.xyz[3 * b + 1], zb = s.xyz[3 * b + 2], qb = s.q[b];
+      const float2 lb = s.lj[b];
+      {
+        const bool ia = (wa0 >> t) & 1u, i4 = (wf0 >> t) & 1u;
+        if (ia || i4) {
+          float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+          float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+          float rx = xb - x0[0], ry = yb - x0[1], rz = zb - x0[2], de;
+          es += pair_energy(rx, ry, rz, q0 * qb * cs, (l0.x + lb.x) * 0.5f, l0.y * lb.y * 4.f * vs,
+                            &de);
+          g0[0] -= rx * de;
+          g0[1] -= ry * de;
+          g0[2] -= rz * de;
+        }
+      }
+      if (two) {
+        const bool ia = (wa1 >> t) & 1u, i4 = (wf1 >> t) & 1u;
+        if (ia || i4) {
+          float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+          float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+          float rx = xb - x1[0], ry = yb - x1[1], rz = zb - x1[2], de;
+          es += pair_energy(rx, ry, rz, q1 * qb * cs, (l1.x + lb.x) * 0.5f, l1.y * lb.y * 4.f * vs,
+                            &de);
+          g1[0] -= rx * de;
+          g1[1] -= ry * de;
+          g1[2] -= rz * de;
+        }
       }
     }
-    if (two) {
-      uint32_t wa = w == 0 ? m1a.x : w == 1 ? m1a.y : w == 2 ? m1a.z : m1a.w;
-      uint32_t wf = w == 0 ? m1f.x : w == 1 ? m1f.y : w == 2 ? m1f.z : m1f.w;
-      bool ia = wa & bit, i4 = wf & bit;
-      if (ia || i4) {
-        float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
-        float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
-        float rx = xb - x1[0], ry = yb - x1[1], rz = zb - x1[2], de;
-        float ep = pair_energy(rx, ry, rz, q1 * qb * cs, (l1.x + lb.x) * 0.5f,
-                               l1.y * lb.y * 4.f * vs, &de);
-        e += 0.5 * (double)ep;
-        g1[0] -= rx * de;
-        g1[1] -= ry * de;
-        g1[2] -= rz * de;
-      }
-    }
+    e += 0.5 * (double)es;
   }
   return e;
 }
