@@ -472,7 +472,7 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
 // scratch (history x 64*VPL floats each).
 template <int VPL>
 __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy, float* hs,
-                            double* s_ro, double* s_al, double* energy_out,
+                            double* hr, double* s_al, double* energy_out,
                             bool* converged_out) {
   const int lane = lane_id();
   const int m = P.history;
@@ -528,7 +528,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
             hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
           }
-          if (lane == 0) s_ro[slot] = 1.0 / ys;
+          hr[slot] = 1.0 / ys;  // every lane stores (and later reads) the same value
           __syncthreads();
         }
         // two-loop recursion (optimizer.cpp:493-507).  q stays in registers;
@@ -543,61 +543,66 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         constexpr int kPre = VPL <= 2 ? 6 : (VPL <= 4 ? 4 : 2);
         const int nk1 = c.nk - 1;
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
+        double br[kPre];
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
-        auto fetch = [&](int h, float* ds, float* dy) {  // h clamped to [0, num_old)
+        auto fetch = [&](int h, float* ds, float* dy, double& rho) {  // h clamped to [0, num_old)
           h = min(max(h, 0), max(num_old - 1, 0));
-          const size_t o = (size_t)(((unsigned)head + (unsigned)h) % (unsigned)m) * NV + lane;
+          const unsigned sl = ((unsigned)head + (unsigned)h) % (unsigned)m;
+          const size_t o = (size_t)sl * NV + lane;
 #pragma unroll
           for (int k = 0; k < VPL; ++k) {
             const int kk = min(k, nk1);
             ds[k] = hs[o + kWave * kk];
             dy[k] = hy[o + kWave * kk];
           }
+          rho = hr[sl];
         };
         const int ngrp = (num_old + kPre - 1) / kPre;
-        const unsigned um = (unsigned)m, uhead = (unsigned)head;
 #pragma unroll
-        for (int j = 0; j < kPre; ++j) fetch(num_old - 1 - j, bs[j], by[j]);
+        for (int j = 0; j < kPre; ++j) fetch(num_old - 1 - j, bs[j], by[j], br[j]);
         for (int grp = 0; grp < ngrp; ++grp) {
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
             const int h = num_old - 1 - (grp * kPre + j);
-            const double sro = s_ro[(uhead + (unsigned)max(h, 0)) % um];
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < VPL; ++k)
               acc += k <= nk1 ? (double)bs[j][k] * (double)qr[k] : 0.0;
-            const double al = wave_sum(acc) * sro * (h >= 0 ? 1.0 : 0.0);
+            double al = wave_sum(acc) * br[j];
+            al = h >= 0 ? al : 0.0;
             s_al[h >= 0 ? h : kMaxHistory] = al;  // every lane stores the same value
             const float alf = (float)al;
 #pragma unroll
             for (int k = 0; k < VPL; ++k) qr[k] -= (k <= nk1 ? alf : 0.f) * by[j][k];
-            fetch(h - kPre, bs[j], by[j]);
+            fetch(h - kPre, bs[j], by[j], br[j]);
           }
         }
         __syncthreads();
         float Hf = (float)H;
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] *= Hf;  // r = H q, kept in qr
+        double ba[kPre];  // alpha of the ring's entries, read ahead like the vectors
 #pragma unroll
-        for (int j = 0; j < kPre; ++j) fetch(j, bs[j], by[j]);
+        for (int j = 0; j < kPre; ++j) {
+          fetch(j, bs[j], by[j], br[j]);
+          ba[j] = s_al[max(min(j, num_old - 1), 0)];
+        }
         for (int grp = 0; grp < ngrp; ++grp) {
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
             const int h = grp * kPre + j;
-            const int hc = min(h, num_old - 1);  // a step past the newest entry reads a valid one
-            const double sro = s_ro[(uhead + (unsigned)hc) % um];
-            const double sal = s_al[hc];
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < VPL; ++k)
               acc += k <= nk1 ? (double)by[j][k] * (double)qr[k] : 0.0;
-            const double be = wave_sum(acc) * sro;
-            const float cf = (float)(sal - be) * (h < num_old ? 1.f : 0.f);
+            const double be = wave_sum(acc) * br[j];
+            float cf = (float)(ba[j] - be);
+            cf = h < num_old ? cf : 0.f;
 #pragma unroll
             for (int k = 0; k < VPL; ++k) qr[k] += bs[j][k] * (k <= nk1 ? cf : 0.f);
-            fetch(h + kPre, bs[j], by[j]);
+            fetch(h + kPre, bs[j], by[j], br[j]);
+            ba[j] = s_al[max(min(h + kPre, num_old - 1), 0)];
           }
         }
 #pragma unroll
@@ -661,7 +666,6 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
                                               int vpl_lo) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
-  __shared__ double s_ro[kMaxHistory];
   __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ __align__(16) float dofl[MODE ? PXD_DOF_LDS_FLOATS : 1];
   __shared__ int s_pose;
@@ -669,6 +673,7 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
   constexpr int NV = kWave * VPL;
   float* hy = A.hist + (size_t)blockIdx.x * lbfgs_slot_floats(A.prm.history);
   float* hs = hy + (size_t)A.prm.history * NV;
+  double* hr = reinterpret_cast<double*>(hs + (size_t)A.prm.history * NV);
   LbfgsCtx<VPL> c;
   c.R = &R;
   c.L = &L;
@@ -711,7 +716,7 @@ __global__ __launch_bounds__(64) void k_lbfgs(DevReceptor R, DevLigands L,
     }
     double energy;
     bool converged;
-    lbfgs_solve<VPL>(c, A.prm, hy, hs, s_ro, s_al, &energy, &converged);
+    lbfgs_solve<VPL>(c, A.prm, hy, hs, hr, s_al, &energy, &converged);
     if (MODE) {  // the pose of the final variables
 #pragma unroll
       for (int k = 0; k < VPL; ++k) {
@@ -869,7 +874,6 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
                                            const DevGrid* __restrict__ G, McArgs A, int vpl_lo) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
-  __shared__ double s_ro[kMaxHistory];
   __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ double s_terms[PXD_NUM_TERMS];
@@ -883,6 +887,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   const int nl = R.n_leaves;
   float* hy = A.hist + (size_t)blockIdx.x * lbfgs_slot_floats(A.hist_m);
   float* hs = hy + (size_t)A.hist_m * NV;
+  double* hr = reinterpret_cast<double*>(hs + (size_t)A.hist_m * NV);
   LbfgsCtx<VPL> c;
   c.R = &R;
   c.L = &L;
@@ -1030,7 +1035,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
       c.nevals = 0;
       double e;
       bool conv;
-      lbfgs_solve<VPL>(c, phase == PH_MUTATE ? P.relax : P.full, hy, hs, s_ro, s_al, &e, &conv);
+      lbfgs_solve<VPL>(c, phase == PH_MUTATE ? P.relax : P.full, hy, hs, hr, s_al, &e, &conv);
       // the single pscore site, at the optimised pose
       for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = x[k];
       for (int k = lane; k < nl; k += kWave) c.s.tors[k] = x[n3 + k];

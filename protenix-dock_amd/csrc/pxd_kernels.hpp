@@ -54,9 +54,10 @@ struct LbfgsArgs {
   float* hist;    // [n_slots][lbfgs_slot_floats(history)]
 };
 
-// Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each.
+// Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each,
+// then rho = 1 / (s.y) per history entry (FP64).
 __host__ __device__ inline size_t lbfgs_slot_floats(int history) {
-  return (size_t)2 * history * kWave * kLbfgsVpl;
+  return (size_t)2 * history * kWave * kLbfgsVpl + (size_t)2 * history;
 }
 
 // L-BFGS size classes: a launch per class takes the poses whose variable
