@@ -477,8 +477,11 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
   }
   __syncthreads();
   if (tab) return e;  // movable-vs-fixed pairs are all inside the leaf tables
-  // movable-vs-fixed pairs: per-slot segments padded to the wave width, the
-  // slot's gradient stays in registers and is reduced once per segment
+  // movable-vs-fixed pairs: per-slot segments padded to the wave width.  A
+  // slot's gradient only enters through its leaf's dE/dtheta = jac . g, so
+  // each lane accumulates jac . g over the leaf's slots (grouped by leaf) and
+  // one wave sum per leaf adds the leaf's share to s.torsg
+  float tl = 0.f;
   for (int m = 0; m < R.n_mov; ++m) {
     float px = s.mov[3 * m], py = s.mov[3 * m + 1], pz = s.mov[3 * m + 2];
     int atom = R.mov_atom[m];
@@ -496,13 +499,12 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
       gz -= rz * de;
     }
     e += em;
-    gx = wave_sum(gx);
-    gy = wave_sum(gy);
-    gz = wave_sum(gz);
-    if (lane == 0) {
-      s.movg[3 * m] += gx;
-      s.movg[3 * m + 1] += gy;
-      s.movg[3 * m + 2] += gz;
+    tl += s.jac[3 * m] * gx + s.jac[3 * m + 1] * gy + s.jac[3 * m + 2] * gz;
+    const int leaf = R.mov_leaf[m];
+    if (m + 1 == R.n_mov || R.mov_leaf[m + 1] != leaf) {
+      const float t = wave_sum(tl);
+      if (lane == 0) s.torsg[leaf] += t;
+      tl = 0.f;
     }
   }
   return e;
@@ -878,7 +880,9 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
       }
     }
   }
-  // movable receptor atoms: explicit pairs, their gradient reduced per slot
+  // movable receptor atoms: explicit pairs; as in receptor_intra, the slots'
+  // gradients enter through jac . g summed per leaf (one wave sum per leaf)
+  float tl = 0.f;
   for (int m = 0; m < R.n_mov; ++m) {
     const float4 mp = s.movp[m];
     const float qm = mp.x;
@@ -900,13 +904,12 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
       gy += ry * de;
       gz += rz * de;
     }
-    gx = wave_sum(gx);
-    gy = wave_sum(gy);
-    gz = wave_sum(gz);
-    if (lane == 0) {
-      s.movg[3 * m] += gx;
-      s.movg[3 * m + 1] += gy;
-      s.movg[3 * m + 2] += gz;
+    tl += s.jac[3 * m] * gx + s.jac[3 * m + 1] * gy + s.jac[3 * m + 2] * gz;
+    const int leaf = R.mov_leaf[m];
+    if (m + 1 == R.n_mov || R.mov_leaf[m + 1] != leaf) {
+      const float t = wave_sum(tl);
+      if (lane == 0) s.torsg[leaf] += t;
+      tl = 0.f;
     }
   }
   if (a0 < n) lig_grad_add(s, a0, g0, 1.f);
@@ -948,6 +951,7 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
 __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                                    const LigInfo& li, PoseLds& s) {
   const int lane = lane_id();
+  for (int k = lane; k < R.n_leaves; k += kWave) s.torsg[k] = 0.f;
   receptor_torsion2xyz(R, s);
   __syncthreads();
   double e = receptor_intra(R, s);
@@ -964,7 +968,7 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
     double te = 0.0, td = 0.0;
     if (R.tab) leaf_lookup(R, k, s.tors[k], &te, &td);
     e += te;
-    s.torsg[k] = (float)td;
+    s.torsg[k] += (float)td;
   }
   __syncthreads();
   for (int m = lane; m < R.n_mov; m += kWave) {
