@@ -753,6 +753,73 @@ __device__ inline double ligand_pairs_masked(const DevLigands& L, const LigInfo&
   return e;
 }
 
+// Ligand non-bonded pairs for ligands of at most 64 atoms, each pair met
+// once: lane a owns atom a; in step k = 1 .. n/2 it takes the pair
+// (a, a + k mod n) — every unordered pair once, the last step of an even n
+// only from its lower half.  Lane a keeps its own gradient in g0; the
+// partner's reaction goes into a travelling accumulator t, which holds atom
+// a + k's sum in step k and moves one lane down the ring (ds_bpermute) after
+// each step; at the end it is sent home.  The partner bits come from the
+// owned atom's masks rotated so that bit k is partner a + k.
+__device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& li,
+                                           const PoseLds& s, float* g0) {
+  const int lane = lane_id(), n = li.n_atoms;
+  const bool act = lane < n;
+  const int a = act ? lane : 0;
+  const uint4* pm = L.pmask + 2ll * li.atom_off;
+  uint64_t ra = 0, rf = 0;
+  if (act) {
+    const uint4 ma = pm[2 * a], mf = pm[2 * a + 1];
+    const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
+    const uint64_t wa = ((uint64_t)ma.y << 32 | ma.x) & full, wf = ((uint64_t)mf.y << 32 | mf.x) & full;
+    // rotate right by a within n bits: bit k <- bit (a + k) mod n
+    ra = a ? ((wa >> a) | (wa << (n - a))) & full : wa;
+    rf = a ? ((wf >> a) | (wf << (n - a))) & full : wf;
+  }
+  const float x0 = s.xyz[3 * a], y0 = s.xyz[3 * a + 1], z0 = s.xyz[3 * a + 2], q0 = s.q[a];
+  const float2 l0 = s.lj[a];
+  const int nxt = act ? (a + 1 == n ? 0 : a + 1) : lane;  // ring successor (source of t)
+  const int K = n / 2;
+  float t[3] = {0.f, 0.f, 0.f}, es = 0.f;
+  int b = a;
+  for (int k = 1; k <= K; ++k) {
+    b = b + 1 == n ? 0 : b + 1;
+    ra >>= 1;
+    rf >>= 1;
+    const bool ia = ra & 1ull, i4 = rf & 1ull;
+    const bool live = act && (ia || i4) && !(2 * k == n && a >= k);
+    if (live) {
+      const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+      const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+      const float2 lb = s.lj[b];
+      const float rx = s.xyz[3 * b] - x0, ry = s.xyz[3 * b + 1] - y0, rz = s.xyz[3 * b + 2] - z0;
+      float de;
+      es += pair_energy(rx, ry, rz, q0 * s.q[b] * cs, (l0.x + lb.x) * 0.5f, l0.y * lb.y * 4.f * vs,
+                        &de);
+      g0[0] -= rx * de;
+      g0[1] -= ry * de;
+      g0[2] -= rz * de;
+      t[0] += rx * de;
+      t[1] += ry * de;
+      t[2] += rz * de;
+    }
+    if (k < K) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        t[d] = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * nxt, __float_as_int(t[d])));
+    }
+  }
+  // t of lane a belongs to atom a + K: atom j's sum sits in lane j - K
+  if (K > 0) {
+    int home = act ? a - K : lane;
+    if (home < 0) home += n;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      g0[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * home, __float_as_int(t[d])));
+  }
+  return (double)es;
+}
+
 // Trilinear value + stored-gradient interpolation (grid.cpp:273-303).
 __device__ inline float4 interp(const float4* map, const long long* off, const float* w) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -959,7 +1026,12 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
   __syncthreads();
   float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
   e += ligand_intra(L, li, s, g0, g1);
-  if (li.pmode) e += ligand_pairs_masked(L, li, s, g0, g1);
+  if (li.pmode) {
+    if (li.n_atoms <= kWave)
+      e += ligand_pairs_ring(L, li, s, g0);
+    else
+      e += ligand_pairs_masked(L, li, s, g0, g1);
+  }
   e += inter(R, G, li, s, g0, g1);
   __syncthreads();
   // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf;
