@@ -127,7 +127,10 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
         le[h] = s.lj[a].y * sqrtf((float)sp.vdw_epsilon_scale[ty]);
       }
     }
+    // per-lane sums in FP32 over blocks of 64 receptor atoms, each block
+    // added to the FP64 totals
     double ec = 0.0, ev = 0.0;
+    float ecf = 0.f, evf = 0.f;
     auto one = [&](const float* pj, float qj, float2 lj) {
       for (int h = 0; h < 2; ++h) {
         if (!has[h]) continue;
@@ -135,25 +138,35 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
         // 1/(d + eps) by one hardware reciprocal square root (saturating at
         // 1/eps for d = 0, as safe_divide, math.h:176-178)
         float rd = fminf(__builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz), 1e20f);
-        if (do_coul) ec += (double)(lq[h] * qj * rd * kCoulomb);
+        if (do_coul) ecf += lq[h] * qj * rd * kCoulomb;
         if (do_vdw) {
           float u = (ls[h] + lj.x) * 0.5f * rd;
           float u2 = u * u;
           float u6 = u2 * u2 * u2;
           float e = (u6 * u6 - u6) * (le[h] * lj.y * 4.f);
-          ev += (double)fminf(fmaxf(e, -cut), cut);
+          evf += fminf(fmaxf(e, -cut), cut);
         }
       }
     };
-    for (int k = 0; k < R.n_fixed; ++k) {  // compacted fixed atoms: one stream
-      float4 v = R.fixed_xyzq[k];
-      float pj[3] = {v.x, v.y, v.z};
-      one(pj, v.w, R.fixed_lj[k]);
+    auto flush = [&]() {
+      ec += (double)ecf;
+      ev += (double)evf;
+      ecf = evf = 0.f;
+    };
+    for (int k0 = 0; k0 < R.n_fixed; k0 += kWave) {  // compacted fixed atoms: one stream
+      const int k1 = min(R.n_fixed, k0 + kWave);
+      for (int k = k0; k < k1; ++k) {
+        float4 v = R.fixed_xyzq[k];
+        float pj[3] = {v.x, v.y, v.z};
+        one(pj, v.w, R.fixed_lj[k]);
+      }
+      flush();
     }
     for (int m = 0; m < R.n_mov; ++m) {
       int j = R.mov_atom[m];
       one(s.mov + 3 * m, R.xyzq[j].w, R.lj[j]);
     }
+    flush();
     t[PXD_TERM_COULOMB] = ec;
     t[PXD_TERM_VDW] = ev * sp.vdw_scale;
   }
@@ -167,13 +180,15 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
       float pi[3];
       lpos(i, pi);
       float s1 = s.lj[i].x;
+      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom, then FP64
       for (int b = 0; b < R.n_hydro; ++b) {
         int j = R.hydro[b];
         float pj[3];
         rpos(j, pj);
         float r1 = (s1 + R.lj[j].x) * sf + dmin;
-        e -= (double)rev_linear(dist3(pi, pj), r1, r1 + dmax);
+        ef += rev_linear(dist3(pi, pj), r1, r1 + dmax);
       }
+      e -= (double)ef;
     }
     t[PXD_TERM_HYDROPHOBIC] = e * sp.hydro_scale;
   }

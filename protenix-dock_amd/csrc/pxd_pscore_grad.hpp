@@ -105,7 +105,7 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
   if (do_coul || do_vdw) {
     const float cut = (float)sp.vdw_cutoff;
     const float wc = w_of(PXD_TERM_COULOMB), wv = w_of(PXD_TERM_VDW) * (float)sp.vdw_scale;
-    float lx[2][3], lq[2], ls[2], le[2], sg[2], es[2];
+    float lx[2][3], lq[2], ls[2], le[2], sg[2], es[2], ies[2];
     int ty[2];
     bool has[2];
     float gx[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
@@ -120,11 +120,15 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         ty[h] = L.type[li.atom_off + a];
         sg[h] = s.lj[a].x;
         es[h] = (float)sp.vdw_epsilon_scale[ty[h]];
+        ies[h] = 1.f / es[h];
         ls[h] = sg[h] * (float)sp.vdw_sigma_scale[ty[h]];
         le[h] = s.lj[a].y * sqrtf(es[h]);
       }
     }
+    // per-lane sums in FP32 over blocks of 64 receptor atoms, each block
+    // added to the FP64 totals
     double ec = 0.0, ev = 0.0;
+    float ecf = 0.f, evf = 0.f, dsf[2] = {0.f, 0.f}, def[2] = {0.f, 0.f};
     auto one = [&](const float* pj, float qj, float2 lj) {
       for (int h = 0; h < 2; ++h) {
         if (!has[h]) continue;
@@ -135,7 +139,7 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         float dedd = 0.f;
         if (do_coul) {
           float e = lq[h] * qj * rd * kCoulomb;
-          ec += (double)e;
+          ecf += e;
           dedd += wc * (-e * rd);
         }
         if (do_vdw) {
@@ -146,14 +150,14 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
           float u12 = u6 * u6;
           float e4 = le[h] * lj.y * 4.f;
           float e = (u12 - u6) * e4;
-          ev += (double)fminf(fmaxf(e, -cut), cut);
+          evf += fminf(fmaxf(e, -cut), cut);
           if (e > -cut && e < cut) {
             // de/du = e4 (12 u^11 - 6 u^5); du/dd = -u/d; du/dsigma_scale = sigma_i / (2 d)
             float dedu_u = e4 * (12.f * u12 - 6.f * u6);  // de/du * u
             dedd += wv * (-dedu_u * rinv);
-            float inv_u = u > 0.f ? 1.f / u : 0.f;
-            dsig[h] += (double)(dedu_u * inv_u * sg[h] * 0.5f * rinv);
-            deps[h] += (double)(0.5f * e / es[h]);
+            float inv_u = u > 0.f ? __builtin_amdgcn_rcpf(u) : 0.f;
+            dsf[h] += dedu_u * inv_u * sg[h] * 0.5f * rinv;
+            def[h] += 0.5f * e * ies[h];
           }
         }
         float f = dedd * rinv;
@@ -162,15 +166,30 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         gx[h][2] += r[2] * f;
       }
     };
-    for (int k = 0; k < R.n_fixed; ++k) {  // compacted fixed atoms: one stream
-      float4 v = R.fixed_xyzq[k];
-      float pj[3] = {v.x, v.y, v.z};
-      one(pj, v.w, R.fixed_lj[k]);
+    auto flush = [&]() {
+      ec += (double)ecf;
+      ev += (double)evf;
+      ecf = evf = 0.f;
+      for (int h = 0; h < 2; ++h) {
+        dsig[h] += (double)dsf[h];
+        deps[h] += (double)def[h];
+        dsf[h] = def[h] = 0.f;
+      }
+    };
+    for (int k0 = 0; k0 < R.n_fixed; k0 += kWave) {  // compacted fixed atoms: one stream
+      const int k1 = min(R.n_fixed, k0 + kWave);
+      for (int k = k0; k < k1; ++k) {
+        float4 v = R.fixed_xyzq[k];
+        float pj[3] = {v.x, v.y, v.z};
+        one(pj, v.w, R.fixed_lj[k]);
+      }
+      flush();
     }
     for (int m = 0; m < R.n_mov; ++m) {
       int j = R.mov_atom[m];
       one(s.mov + 3 * m, R.xyzq[j].w, R.lj[j]);
     }
+    flush();
     t[PXD_TERM_COULOMB] = ec;
     t[PXD_TERM_VDW] = ev * sp.vdw_scale;
     const double wvd = do_vdw ? sp.vdw_coef * sp.vdw_scale : 0.0;
@@ -194,6 +213,7 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
       float pi[3], gi[3] = {0.f, 0.f, 0.f};
       lpos(i, pi);
       float s1 = s.lj[i].x;
+      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom, then FP64
       for (int b = 0; b < R.n_hydro; ++b) {
         int j = R.hydro[b];
         float pj[3];
@@ -201,12 +221,13 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         float r1 = (s1 + R.lj[j].x) * sf + dmin;
         float r[3] = {pi[0] - pj[0], pi[1] - pj[1], pi[2] - pj[2]};
         float d = sqrtf(dot3(r, r));
-        e -= (double)rev_linear(d, r1, r1 + dmax);
-        float f = -w * rev_linear_d(d, r1, r1 + dmax) / d;
+        ef += rev_linear(d, r1, r1 + dmax);
+        float f = -w * rev_linear_d(d, r1, r1 + dmax) * __builtin_amdgcn_rcpf(d);
         gi[0] += r[0] * f;
         gi[1] += r[1] * f;
         gi[2] += r[2] * f;
       }
+      e -= (double)ef;
       gadd(i, gi, 1.f);
     }
     t[PXD_TERM_HYDROPHOBIC] = e * sp.hydro_scale;
