@@ -176,12 +176,20 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         dsf[h] = def[h] = 0.f;
       }
     };
-    for (int k0 = 0; k0 < R.n_fixed; k0 += kWave) {  // compacted fixed atoms: one stream
+    // compacted fixed atoms: one stream, the next atom's (uniform) loads
+    // issued before the current atom is evaluated
+    float4 vn = R.n_fixed > 0 ? R.fixed_xyzq[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 ln = R.n_fixed > 0 ? R.fixed_lj[0] : make_float2(0.f, 0.f);
+    for (int k0 = 0; k0 < R.n_fixed; k0 += kWave) {
       const int k1 = min(R.n_fixed, k0 + kWave);
       for (int k = k0; k < k1; ++k) {
-        float4 v = R.fixed_xyzq[k];
+        const float4 v = vn;
+        const float2 lj = ln;
+        const int kn = min(k + 1, R.n_fixed - 1);
+        vn = R.fixed_xyzq[kn];
+        ln = R.fixed_lj[kn];
         float pj[3] = {v.x, v.y, v.z};
-        one(pj, v.w, R.fixed_lj[k]);
+        one(pj, v.w, lj);
       }
       flush();
     }
