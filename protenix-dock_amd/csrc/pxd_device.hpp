@@ -821,11 +821,16 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
 }
 
 // Trilinear value + stored-gradient interpolation (grid.cpp:273-303).
-__device__ inline float4 interp(const float4* map, const long long* off, const float* w) {
+__device__ inline float4 interp(const float4* map, const int* off, const float* w) {
+  // the maps live in global memory: global (not flat) loads, which do not
+  // hold up the waits for LDS results
+  typedef float vf4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(1))) vf4 gvf4;
+  const gvf4* gm = (const gvf4*)map;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
-    float4 v = map[off[c]];
+    const vf4 v = gm[off[c]];
     acc.x += v.x * w[c];
     acc.y += v.y * w[c];
     acc.z += v.z * w[c];
@@ -874,44 +879,32 @@ __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float
     hit = false;
   }
   // layout 0: point-major maps, the corners 1, nz, ny*nz points apart;
-  // layout 1: the cell's 8 corners contiguous (one 128-byte line per map)
-  const float4* m0;
-  const float4* m1;
-  long long off[8];
+  // layout 1: the cell's 8 corners contiguous (one 128-byte line per map).
+  // One base address per map; the corners are constant (layout 1) or shared
+  // 32-bit (layout 0) offsets from it.
+  float4 c, v;
   if (G.layout) {
     const long long cell =
         (((long long)lo[0] * (G.np[1] - 1) + lo[1]) * (G.np[2] - 1) + lo[2]) * 8;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) off[k] = cell + k;
-    m0 = G.cmaps;
-    m1 = G.cmaps + G.C * 8 * slot;
+    const float4* p0 = G.cmaps + cell;
+    const float4* p1 = G.cmaps + G.C * 8 * slot + cell;
+    const int o[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    c = interp(p0, o, w);
+    v = interp(p1, o, w);
   } else {
-    const long long ny = G.np[1], nz = G.np[2];
-    const long long base = ((long long)lo[0] * ny + lo[1]) * nz + lo[2];
-    off[0] = base;
-    off[1] = base + ny * nz;
-    off[2] = base + nz;
-    off[3] = base + ny * nz + nz;
-    off[4] = base + 1;
-    off[5] = base + ny * nz + 1;
-    off[6] = base + nz + 1;
-    off[7] = base + ny * nz + nz + 1;
-    m0 = G.maps;
-    m1 = G.maps + G.P * slot;
+    const int nz = G.np[2], nyz = G.np[1] * G.np[2];
+    const long long base = ((long long)lo[0] * G.np[1] + lo[1]) * nz + lo[2];
+    const float4* p0 = G.maps + base;
+    const float4* p1 = G.maps + G.P * slot + base;
+    const int o[8] = {0, nyz, nz, nyz + nz, 1, nyz + 1, nz + 1, nyz + nz + 1};
+    c = interp(p0, o, w);
+    v = interp(p1, o, w);
   }
-  float4 c = interp(m0, off, w);
-  *e += (double)(c.x * q);
+  *e += (double)(c.x * q + v.x);
   if (hit) {
-    g[0] += c.y * q;
-    g[1] += c.z * q;
-    g[2] += c.w * q;
-  }
-  float4 v = interp(m1, off, w);
-  *e += (double)v.x;
-  if (hit) {
-    g[0] += v.y;
-    g[1] += v.z;
-    g[2] += v.w;
+    g[0] += c.y * q + v.y;
+    g[1] += c.z * q + v.z;
+    g[2] += c.w * q + v.w;
   }
 }
 
