@@ -765,6 +765,23 @@ void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                      G, c, 0);
 }
 
+// Ring centroids / normals of the pscore leaves in dynamic LDS, sized by the
+// receptor's ring count at launch (ring_lds_bytes) rather than the maximum.
+extern __shared__ __align__(16) float ring_lds[];
+
+__device__ inline PscoreLds carve_ring_lds(float* base, int n_rec_rings) {
+  PscoreLds ps;
+  ps.rc = base;
+  ps.rn = base + 3 * n_rec_rings;
+  ps.lc = base + 6 * n_rec_rings;
+  ps.ln = ps.lc + 3 * kMaxLigRings;
+  return ps;
+}
+
+static size_t ring_lds_bytes(const DevReceptor& R) {
+  return sizeof(float) * 6 * (size_t)(R.n_ring5 + R.n_ring6 + kMaxLigRings);
+}
+
 // ============================================================== K5: pscore
 // root_scorer::combine for a batch of poses, one wavefront per pose.
 
@@ -776,14 +793,10 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
                                                double* __restrict__ score,
                                                double* __restrict__ terms) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
-  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ double s_terms[PXD_NUM_TERMS];
   PoseLds s = carve_pose_lds(lds);
   PscoreLds ps;
-  ps.rc = rings;
-  ps.rn = rings + 3 * kMaxRecRings;
-  ps.lc = rings + 6 * kMaxRecRings;
-  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
   const int p = xcd_block(blockIdx.x, gridDim.x);
   if (p >= n_poses) return;
@@ -805,7 +818,8 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
 void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                    const PscoreArgs& a, hipStream_t st) {
   if (a.n_poses <= 0) return;
-  hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
+  hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), ring_lds_bytes(R), st, R, L, sp,
+                     a.n_poses,
                      a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms);
 }
 
@@ -820,15 +834,11 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
                                                     float* __restrict__ dxyz,
                                                     double* __restrict__ dparams) {
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
-  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ float s_gx[kMaxAtoms * 3];
   __shared__ double s_gp[PXD_NUM_SCORE_PARAMS];
   PoseLds s = carve_pose_lds(lds);
   PscoreLds ps;
-  ps.rc = rings;
-  ps.rn = rings + 3 * kMaxRecRings;
-  ps.lc = rings + 6 * kMaxRecRings;
-  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
   ps.terms = nullptr;
   PscoreGradLds g{s_gx, s_gp};
   const int p = xcd_block(blockIdx.x, gridDim.x);
@@ -855,7 +865,8 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
 void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                         const PscoreArgs& a, float* dxyz, double* dparams, hipStream_t st) {
   if (a.n_poses <= 0) return;
-  hipLaunchKernelGGL(k_pscore_grad, dim3(a.n_poses), dim3(kWave), 0, st, R, L, sp, a.n_poses,
+  hipLaunchKernelGGL(k_pscore_grad, dim3(a.n_poses), dim3(kWave), ring_lds_bytes(R), st, R, L,
+                     sp, a.n_poses,
                      a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, dxyz, dparams);
 }
 
@@ -875,7 +886,6 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
-  __shared__ float rings[(kMaxRecRings + kMaxLigRings) * 6];
   __shared__ double s_terms[PXD_NUM_TERMS];
   __shared__ float apl[PXD_APPLY_LDS_FLOATS];
   __shared__ float curx[kMaxAtoms * 3 + kMaxLeaves];
@@ -896,10 +906,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   c.v = vpool;
   c.mode = 0;
   PscoreLds ps;
-  ps.rc = rings;
-  ps.rn = rings + 3 * kMaxRecRings;
-  ps.lc = rings + 6 * kMaxRecRings;
-  ps.ln = ps.lc + 3 * kMaxLigRings;
+  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
   ApplyLds ap = carve_apply_lds(apl);
   BoxF box;
@@ -1115,13 +1122,14 @@ void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, cons
   // one launch per L-BFGS size class (the walker's ligand decides), each
   // with its own queue head: a.counter[0..kLbfgsClasses)
   McArgs c = a;
-  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[2]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c,
+  const size_t dyn = ring_lds_bytes(R);
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[2]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c,
                      kLbfgsClassVpl[1]);
   c.counter = a.counter + 1;
-  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[1]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c,
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[1]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c,
                      kLbfgsClassVpl[0]);
   c.counter = a.counter + 2;
-  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[0]>, dim3(n_slots), dim3(kWave), 0, st, R, L, G, c, 0);
+  hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[0]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c, 0);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {
