@@ -302,6 +302,11 @@ constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
    pxd::kMaxLeaves * 2 +                                                      \
    (pxd::kMaxAtoms * 3 > pxd::kStageSlots * 4 ? pxd::kMaxAtoms * 3 : pxd::kStageSlots * 4))
 
+// The pscore kernels use neither the gradient array nor the stage: their
+// pose LDS ends before them.
+#define PXD_POSE_LDS_FLOATS_SCORE_ONLY \
+  (pxd::kMaxAtoms * 3 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 + pxd::kMaxLeaves * 2)
+
 __device__ inline PoseLds carve_pose_lds(float* base) {
   PoseLds s;
   s.xyz = base;

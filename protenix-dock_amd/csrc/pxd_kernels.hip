@@ -792,7 +792,7 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
                                                const float* __restrict__ tors,
                                                double* __restrict__ score,
                                                double* __restrict__ terms) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS_SCORE_ONLY];
   __shared__ double s_terms[PXD_NUM_TERMS];
   PoseLds s = carve_pose_lds(lds);
   PscoreLds ps;
@@ -833,7 +833,7 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
                                                     double* __restrict__ score,
                                                     float* __restrict__ dxyz,
                                                     double* __restrict__ dparams) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
+  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS_SCORE_ONLY];
   __shared__ float s_gx[kMaxAtoms * 3];
   __shared__ double s_gp[PXD_NUM_SCORE_PARAMS];
   PoseLds s = carve_pose_lds(lds);
