@@ -1091,6 +1091,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   b->h_natoms = new int[nlig];
   b->h_ntors = new int[nlig];
   memcpy(b->h_natoms, natoms.data(), sizeof(int) * nlig);
+  b->max_atoms = *std::max_element(natoms.begin(), natoms.end());
   memcpy(b->h_ntors, ntors.data(), sizeof(int) * nlig);
   memcpy(b->type_used, used, sizeof(used));
   b->has_trees = trees ? 1 : 0;
@@ -1136,7 +1137,8 @@ pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* g,
     return fail(PXD_ERR_ARG, "score: tors_grad required when the receptor has leaves");
   pxd_status st = check_grid_types(g, b);
   if (st != PXD_OK) return st;
-  ScoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, energy, lig_grad, tors_grad};
+  ScoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, energy, lig_grad, tors_grad,
+              b->max_atoms};
   launch_score(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, static_cast<hipStream_t>(stream));
   HIP_TRY(hipGetLastError());
   return PXD_OK;

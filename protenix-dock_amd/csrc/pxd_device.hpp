@@ -307,21 +307,50 @@ constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
 #define PXD_POSE_LDS_FLOATS_SCORE_ONLY \
   (pxd::kMaxAtoms * 3 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 + pxd::kMaxLeaves * 2)
 
-__device__ inline PoseLds carve_pose_lds(float* base) {
+// Offsets (floats) of the per-pose LDS arrays for ligands of up to ma atoms,
+// nm movable receptor slots and nlv leaves; every array starts on a 16-byte
+// boundary.  With the maxima this is the PXD_POSE_LDS_FLOATS layout.
+struct PoseDims {
+  int q, lj, slot, mov, movg, jac, tors, torsg, movp, grad, total;
+};
+
+__host__ __device__ constexpr PoseDims pose_dims(int ma, int nm, int nlv) {
+  auto r4 = [](int x) { return (x + 3) & ~3; };
+  PoseDims d{};
+  d.q = r4(3 * ma);
+  d.lj = d.q + r4(ma);
+  d.slot = d.lj + r4(2 * ma);
+  d.mov = d.slot + r4(ma);
+  d.movg = d.mov + r4(3 * nm);
+  d.jac = d.movg + r4(3 * nm);
+  d.tors = d.jac + r4(3 * nm);
+  d.torsg = d.tors + r4(nlv);
+  d.movp = d.torsg + r4(nlv);
+  d.grad = d.movp + 4 * nm;
+  const int g = r4(3 * ma) > 4 * kStageSlots ? r4(3 * ma) : 4 * kStageSlots;
+  d.total = d.grad + g;
+  return d;
+}
+
+__device__ inline PoseLds carve_pose_lds(float* base, const PoseDims& d) {
   PoseLds s;
   s.xyz = base;
-  s.q = s.xyz + kMaxAtoms * 3;
-  s.lj = reinterpret_cast<float2*>(s.q + kMaxAtoms);
-  s.slot = reinterpret_cast<int*>(s.q + kMaxAtoms * 3);
-  s.mov = s.q + kMaxAtoms * 4;
-  s.movg = s.mov + kMaxMovable * 3;
-  s.jac = s.movg + kMaxMovable * 3;
-  s.tors = s.jac + kMaxMovable * 3;
-  s.torsg = s.tors + kMaxLeaves;
-  s.movp = reinterpret_cast<float4*>(s.torsg + kMaxLeaves);
-  s.grad = reinterpret_cast<float*>(s.movp + kMaxMovable);
+  s.q = base + d.q;
+  s.lj = reinterpret_cast<float2*>(base + d.lj);
+  s.slot = reinterpret_cast<int*>(base + d.slot);
+  s.mov = base + d.mov;
+  s.movg = base + d.movg;
+  s.jac = base + d.jac;
+  s.tors = base + d.tors;
+  s.torsg = base + d.torsg;
+  s.movp = reinterpret_cast<float4*>(base + d.movp);
+  s.grad = base + d.grad;
   s.stage = reinterpret_cast<float4*>(s.grad);
   return s;
+}
+
+__device__ inline PoseLds carve_pose_lds(float* base) {
+  return carve_pose_lds(base, pose_dims(kMaxAtoms, kMaxMovable, kMaxLeaves));
 }
 
 // Per-atom parameters of a ligand into LDS (pose independent).

@@ -125,6 +125,12 @@ void launch_grid_build(const GridBuildArgs& a, int types_per_pass, hipStream_t s
     grid_passes<16>(a, st);
 }
 
+// Dynamic LDS of a launch (sized by the host per launch): the pose arrays of
+// k_score, the pscore ring geometry of the pscore kernels and MC walkers.
+extern __shared__ __align__(16) float dyn_lds[];
+static_assert(pose_dims(kMaxAtoms, kMaxMovable, kMaxLeaves).total == PXD_POSE_LDS_FLOATS,
+              "pose LDS layout at the maxima");
+
 // ============================================================== K2: score+grad
 // One wavefront (= one block) per pose.
 
@@ -136,9 +142,9 @@ __global__ __launch_bounds__(64) void k_score(DevReceptor R, DevLigands L,
                                               const float* __restrict__ tors,
                                               double* __restrict__ energy,
                                               float* __restrict__ lig_grad,
-                                              float* __restrict__ tors_grad) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
-  PoseLds s = carve_pose_lds(lds);
+                                              float* __restrict__ tors_grad, int max_atoms) {
+  // pose arrays sized for this launch's largest ligand and this receptor
+  PoseLds s = carve_pose_lds(dyn_lds, pose_dims(max_atoms, R.n_mov, R.n_leaves));
   const int p = xcd_block(blockIdx.x, gridDim.x);
   if (p >= n_poses) return;
   const int lane = lane_id();
@@ -160,8 +166,11 @@ __global__ __launch_bounds__(64) void k_score(DevReceptor R, DevLigands L,
 void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const ScoreArgs& a, hipStream_t st) {
   if (a.n_poses <= 0) return;
-  hipLaunchKernelGGL(k_score, dim3(a.n_poses), dim3(kWave), 0, st, R, L, G, a.n_poses,
-                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.energy, a.lig_grad, a.tors_grad);
+  const int ma = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  const size_t lds = sizeof(float) * pose_dims(ma, R.n_mov, R.n_leaves).total;
+  hipLaunchKernelGGL(k_score, dim3(a.n_poses), dim3(kWave), lds, st, R, L, G, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.energy, a.lig_grad, a.tors_grad,
+                     ma);
 }
 
 // ============================================================== K3: L-BFGS
@@ -767,7 +776,6 @@ void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
 
 // Ring centroids / normals of the pscore leaves in dynamic LDS, sized by the
 // receptor's ring count at launch (ring_lds_bytes) rather than the maximum.
-extern __shared__ __align__(16) float ring_lds[];
 
 __device__ inline PscoreLds carve_ring_lds(float* base, int n_rec_rings) {
   PscoreLds ps;
@@ -796,7 +804,7 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
   __shared__ double s_terms[PXD_NUM_TERMS];
   PoseLds s = carve_pose_lds(lds);
   PscoreLds ps;
-  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
   const int p = xcd_block(blockIdx.x, gridDim.x);
   if (p >= n_poses) return;
@@ -838,7 +846,7 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
   __shared__ double s_gp[PXD_NUM_SCORE_PARAMS];
   PoseLds s = carve_pose_lds(lds);
   PscoreLds ps;
-  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
   ps.terms = nullptr;
   PscoreGradLds g{s_gx, s_gp};
   const int p = xcd_block(blockIdx.x, gridDim.x);
@@ -906,7 +914,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   c.v = vpool;
   c.mode = 0;
   PscoreLds ps;
-  ps = carve_ring_lds(ring_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
   ApplyLds ap = carve_apply_lds(apl);
   BoxF box;

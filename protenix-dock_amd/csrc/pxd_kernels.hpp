@@ -32,6 +32,7 @@ struct ScoreArgs {
   double* energy;
   float* lig_grad;
   float* tors_grad;
+  int max_atoms;  // largest ligand of the batch (sizes the per-pose LDS); <= 0: the maximum
 };
 
 struct LbfgsParamsDev {

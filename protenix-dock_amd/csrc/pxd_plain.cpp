@@ -41,7 +41,7 @@ struct PlainEval {
     hipError_t e = hipMemcpyAsync(d_xyz, hx.data(), sizeof(float) * n3, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && nt)
       e = hipMemcpyAsync(d_tors, hx.data() + n3, sizeof(float) * nt, hipMemcpyHostToDevice, st);
-    ScoreArgs a{1, d_lig, d_off, d_xyz, d_tors, d_e, d_g, d_tg};
+    ScoreArgs a{1, d_lig, d_off, d_xyz, d_tors, d_e, d_g, d_tg, b->max_atoms};
     if (e == hipSuccess) {
       launch_score(r->dev, b->dev, G, a, st);
       e = hipGetLastError();
