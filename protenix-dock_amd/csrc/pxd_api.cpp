@@ -1197,6 +1197,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   a.converged = converged;
   a.counter = g_ws.counter;
   a.hist = g_ws.hist;
+  a.max_atoms = b->max_atoms;
   launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
   HIP_TRY(hipGetLastError());
   // the shared workspace may be reused by the next call: finish this one

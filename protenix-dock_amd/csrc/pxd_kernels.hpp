@@ -53,6 +53,7 @@ struct LbfgsArgs {
   int* converged;
   int* counter;   // work-queue heads (one per size class), zero before launch
   float* hist;    // [n_slots][lbfgs_slot_floats(history)]
+  int max_atoms;  // largest ligand of the batch (sizes the per-pose LDS); <= 0: the maximum
 };
 
 // Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each,
