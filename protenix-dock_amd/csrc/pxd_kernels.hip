@@ -543,7 +543,9 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
           hr[slot] = 1.0 / ys;  // every lane stores (and later reads) the same value
           __syncthreads();
         }
-        // two-loop recursion (optimizer.cpp:493-507).  q stays in registers;
+        // two-loop recursion (optimizer.cpp:493-507), its dot products in FP32
+        // (FP64 stays for s.y, y.y, the Armijo test and the line search's
+        // scalars).  q stays in registers;
         // history entries are loaded kPre steps ahead of their use into a
         // ring of register buffers (statically indexed by unrolling kPre
         // steps).  Every load is unconditional — indices are clamped, and a
@@ -577,11 +579,11 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
             const int h = num_old - 1 - (grp * kPre + j);
-            double acc = 0.0;
+            float acc = 0.f;
 #pragma unroll
             for (int k = 0; k < VPL; ++k)
-              acc += k <= nk1 ? (double)bs[j][k] * (double)qr[k] : 0.0;
-            double al = wave_sum(acc) * br[j];
+              acc += k <= nk1 ? bs[j][k] * qr[k] : 0.f;
+            double al = (double)wave_sum(acc) * br[j];
             al = h >= 0 ? al : 0.0;
             s_al[h >= 0 ? h : kMaxHistory] = al;  // every lane stores the same value
             const float alf = (float)al;
@@ -604,11 +606,11 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 #pragma unroll
           for (int j = 0; j < kPre; ++j) {
             const int h = grp * kPre + j;
-            double acc = 0.0;
+            float acc = 0.f;
 #pragma unroll
             for (int k = 0; k < VPL; ++k)
-              acc += k <= nk1 ? (double)by[j][k] * (double)qr[k] : 0.0;
-            const double be = wave_sum(acc) * br[j];
+              acc += k <= nk1 ? by[j][k] * qr[k] : 0.f;
+            const double be = (double)wave_sum(acc) * br[j];
             float cf = (float)(ba[j] - be);
             cf = h < num_old ? cf : 0.f;
 #pragma unroll
