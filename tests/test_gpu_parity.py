@@ -163,6 +163,47 @@ def test_gpu_score_grad_matches_oracle_with_maps(world):
     assert not bad, bad[:5]
 
 
+def test_gpu_pair_schedules_at_size_boundaries(world):
+    """Ligands of 3, 5, 33, 63, 64 atoms take the ring pair schedule (each
+    pair once, reactions in a travelling accumulator); 65 and 66 atoms take
+    the two-atoms-per-lane sweep.  Energies and gradients against the oracle
+    on the same maps, poses near the pocket (many close pairs)."""
+    picks = [(5001, 2, 3), (5013, 3, 5), (5007, 28, 33), (5004, 54, 63), (5018, 54, 64),
+             (5000, 54, 65), (5002, 54, 66)]
+    ligs = []
+    for seed, nh, n in picks:
+        m = synth.make_ligand(seed, n_heavy=nh, n_torsions=min(3, max(0, nh - 3)), n_rings=0)
+        assert m.n_atoms == n, (seed, nh, m.n_atoms)
+        ligs.append(m)
+    types = sorted(set(int(t) for m in ligs for t in m.vdw_type))
+    G = lib.Grid(world["R"], world["box"], types)
+    og = O.Grid(world["box"], G.types, G.download().astype(np.float64), 1e6,
+                world["rec"].leaves.movable())
+    rng = np.random.default_rng(77)
+    pose_lig, xyz = [], []
+    for l, lig in enumerate(ligs):
+        for _ in range(3):
+            c = world["rec"].pocket_center + rng.normal(size=3) * 2.0
+            R = synth.quat_to_rot(synth.random_quaternion(rng))
+            t = rng.uniform(-math.pi, math.pi, lig.tree.n_torsions)
+            xyz.append(_f32(synth.ligand_apply(lig.tree, lig.xyz[0], c, R, t)))
+            pose_lig.append(l)
+    tors = _f32(rng.uniform(-math.pi, math.pi, (len(xyz), world["rec"].leaves.n_leaves)))
+    B = lib.LigandBatch(ligs)
+    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    e, g, tg = lib.score_grad(world["R"], G, P)
+    e, tg = e.cpu().numpy(), tg.cpu().numpy()
+    systems = [O.System(world["rec"], lig, og) for lig in ligs]
+    bad = []
+    for p in range(P.n_poses):
+        re, rg, rtg = systems[pose_lig[p]].energy(xyz[p], tors[p])
+        full = np.concatenate([P.xyz_of(p, g).ravel(), tg[p]])
+        rfull = np.concatenate([rg.ravel(), rtg])
+        if not (_close_e(e[p], re) and _grad_ok(full, rfull)):
+            bad.append((ligs[pose_lig[p]].n_atoms, e[p], re, np.linalg.norm(full - rfull)))
+    assert not bad, bad
+
+
 def test_gpu_score_grad_brute_force_matches_oracle(world):
     """No grid: inter term by explicit pairs over every receptor atom (interaction.cpp:241-254)."""
     pose_lig, xyz, tors = _poses(world, 4, 12, spread=2.0)
