@@ -204,6 +204,37 @@ def test_gpu_pair_schedules_at_size_boundaries(world):
     assert not bad, bad
 
 
+def test_gpu_pair_list_form_when_masks_cannot_hold_the_list(world):
+    """A pair list that names a pair twice cannot be held by partner masks:
+    the ligand keeps the list form (each listed entry evaluated, gradients by
+    LDS atomics).  The oracle walks the same list."""
+    m = synth.make_ligand(5007, n_heavy=28, n_torsions=3, n_rings=0)
+    m.pairs = np.concatenate([m.pairs, m.pairs[:5]])
+    ligs = [m]
+    types = sorted(set(int(t) for t in m.vdw_type))
+    G = lib.Grid(world["R"], world["box"], types)
+    og = O.Grid(world["box"], G.types, G.download().astype(np.float64), 1e6,
+                world["rec"].leaves.movable())
+    rng = np.random.default_rng(78)
+    xyz = []
+    for _ in range(4):
+        c = world["rec"].pocket_center + rng.normal(size=3) * 2.0
+        R = synth.quat_to_rot(synth.random_quaternion(rng))
+        t = rng.uniform(-math.pi, math.pi, m.tree.n_torsions)
+        xyz.append(_f32(synth.ligand_apply(m.tree, m.xyz[0], c, R, t)))
+    tors = _f32(rng.uniform(-math.pi, math.pi, (len(xyz), world["rec"].leaves.n_leaves)))
+    P = lib.PoseSet(lib.LigandBatch(ligs), [0] * len(xyz), xyz, 8, tors=tors)
+    e, g, tg = lib.score_grad(world["R"], G, P)
+    e, tg = e.cpu().numpy(), tg.cpu().numpy()
+    sysm = O.System(world["rec"], m, og)
+    for p in range(P.n_poses):
+        re, rg, rtg = sysm.energy(xyz[p], tors[p])
+        full = np.concatenate([P.xyz_of(p, g).ravel(), tg[p]])
+        rfull = np.concatenate([rg.ravel(), rtg])
+        assert _close_e(e[p], re), (p, e[p], re)
+        assert _grad_ok(full, rfull), (p, np.linalg.norm(full - rfull))
+
+
 def test_gpu_score_grad_brute_force_matches_oracle(world):
     """No grid: inter term by explicit pairs over every receptor atom (interaction.cpp:241-254)."""
     pose_lig, xyz, tors = _poses(world, 4, 12, spread=2.0)
