@@ -1231,7 +1231,7 @@ pxd_status pxd_pscore_grad_batch(const pxd_receptor* r, const pxd_ligand_batch* 
     return fail(PXD_ERR_ARG, "pscore_grad: null pose array");
   if (r->dev.n_ring5 + r->dev.n_ring6 > 256)
     return fail(PXD_ERR_RANGE, "pscore_grad: receptor has more than 256 pi rings");
-  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, nullptr};
+  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, nullptr, b->max_atoms};
   launch_pscore_grad(r->dev, b->dev, *sp, a, dscore_dxyz, dscore_dparams,
                      static_cast<hipStream_t>(stream));
   HIP_TRY(hipGetLastError());
@@ -1248,7 +1248,7 @@ pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
   if (!pose_lig || !pose_atom_off || !xyz || !score) return fail(PXD_ERR_ARG, "pscore: null pose array");
   if (r->dev.n_ring5 + r->dev.n_ring6 > 256)
     return fail(PXD_ERR_RANGE, "pscore: receptor has more than 256 pi rings");
-  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, terms};
+  PscoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, score, terms, b->max_atoms};
   launch_pscore(r->dev, b->dev, *sp, a, static_cast<hipStream_t>(stream));
   HIP_TRY(hipGetLastError());
   return PXD_OK;

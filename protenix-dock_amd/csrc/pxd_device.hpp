@@ -302,11 +302,6 @@ constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
    pxd::kMaxLeaves * 2 +                                                      \
    (pxd::kMaxAtoms * 3 > pxd::kStageSlots * 4 ? pxd::kMaxAtoms * 3 : pxd::kStageSlots * 4))
 
-// The pscore kernels use neither the gradient array nor the stage: their
-// pose LDS ends before them.
-#define PXD_POSE_LDS_FLOATS_SCORE_ONLY \
-  (pxd::kMaxAtoms * 3 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 + pxd::kMaxLeaves * 2)
-
 // Offsets (floats) of the per-pose LDS arrays for ligands of up to ma atoms,
 // nm movable receptor slots and nlv leaves; every array starts on a 16-byte
 // boundary.  With the maxima this is the PXD_POSE_LDS_FLOATS layout.

@@ -808,12 +808,14 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
                                                const float* __restrict__ xyz,
                                                const float* __restrict__ tors,
                                                double* __restrict__ score,
-                                               double* __restrict__ terms) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS_SCORE_ONLY];
+                                               double* __restrict__ terms, int max_atoms) {
   __shared__ double s_terms[PXD_NUM_TERMS];
-  PoseLds s = carve_pose_lds(lds);
+  // dynamic LDS: the pose arrays (no gradient area) sized for the batch's
+  // largest ligand, then the ring geometry
+  const PoseDims pd = pose_dims(max_atoms, R.n_mov, R.n_leaves);
+  PoseLds s = carve_pose_lds(dyn_lds, pd);
   PscoreLds ps;
-  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds + pd.grad, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
   const int p = xcd_block(blockIdx.x, gridDim.x);
   if (p >= n_poses) return;
@@ -835,9 +837,10 @@ __global__ __launch_bounds__(64) void k_pscore(DevReceptor R, DevLigands L, pxd_
 void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                    const PscoreArgs& a, hipStream_t st) {
   if (a.n_poses <= 0) return;
-  hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), ring_lds_bytes(R), st, R, L, sp,
-                     a.n_poses,
-                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms);
+  const int ma = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  const size_t lds = sizeof(float) * pose_dims(ma, R.n_mov, R.n_leaves).grad + ring_lds_bytes(R);
+  hipLaunchKernelGGL(k_pscore, dim3(a.n_poses), dim3(kWave), lds, st, R, L, sp, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, a.terms, ma);
 }
 
 // pscore with its derivatives (pxd_pscore_grad.hpp), one wavefront per pose.
@@ -849,13 +852,13 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
                                                     const float* __restrict__ tors,
                                                     double* __restrict__ score,
                                                     float* __restrict__ dxyz,
-                                                    double* __restrict__ dparams) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS_SCORE_ONLY];
+                                                    double* __restrict__ dparams, int max_atoms) {
   __shared__ float s_gx[kMaxAtoms * 3];
   __shared__ double s_gp[PXD_NUM_SCORE_PARAMS];
-  PoseLds s = carve_pose_lds(lds);
+  const PoseDims pd = pose_dims(max_atoms, R.n_mov, R.n_leaves);
+  PoseLds s = carve_pose_lds(dyn_lds, pd);
   PscoreLds ps;
-  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds + pd.grad, R.n_ring5 + R.n_ring6);
   ps.terms = nullptr;
   PscoreGradLds g{s_gx, s_gp};
   const int p = xcd_block(blockIdx.x, gridDim.x);
@@ -882,9 +885,10 @@ __global__ __launch_bounds__(64) void k_pscore_grad(DevReceptor R, DevLigands L,
 void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                         const PscoreArgs& a, float* dxyz, double* dparams, hipStream_t st) {
   if (a.n_poses <= 0) return;
-  hipLaunchKernelGGL(k_pscore_grad, dim3(a.n_poses), dim3(kWave), ring_lds_bytes(R), st, R, L,
-                     sp, a.n_poses,
-                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, dxyz, dparams);
+  const int ma = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  const size_t lds = sizeof(float) * pose_dims(ma, R.n_mov, R.n_leaves).grad + ring_lds_bytes(R);
+  hipLaunchKernelGGL(k_pscore_grad, dim3(a.n_poses), dim3(kWave), lds, st, R, L, sp, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.score, dxyz, dparams, ma);
 }
 
 // ============================================================== K4: Monte-Carlo search

@@ -76,6 +76,7 @@ struct PscoreArgs {
   const float* tors;
   double* score;
   double* terms;
+  int max_atoms;  // largest ligand of the batch (sizes the per-pose LDS); <= 0: the maximum
 };
 
 struct McParamsDev {
