@@ -844,7 +844,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   std::vector<int2> dons;
   std::vector<uint4> pmask;
   std::vector<uint32_t> bslot, aslot, dslot, gat;
-  std::vector<int> natoms(nlig), ntors(nlig);
+  std::vector<int> natoms(nlig), ntors(nlig), nfrags(nlig);
   uint8_t used[PXD_NUM_VDW_TYPES] = {0};
   for (int l = 0; l < nlig; ++l) {
     const pxd_mol& m = ligs[l];
@@ -983,9 +983,21 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
         int s = T.frag_start[f], e = T.frag_end[f], par = T.frag_parent[f];
         if (s < 0 || e > n || s >= e || par >= T.n_frags)
           return fail(PXD_ERR_CONF, "ligand %d: bad fragment %d", l, f);
+        if (par >= 0 && (T.frag_bond[2 * f] < 0 || T.frag_bond[2 * f] >= n ||
+                         T.frag_bond[2 * f + 1] < 0 || T.frag_bond[2 * f + 1] >= n))
+          return fail(PXD_ERR_CONF, "ligand %d: fragment %d bond atom out of range", l, f);
         frag.insert(frag.end(), {s, e, par, T.frag_bond[2 * f], T.frag_bond[2 * f + 1], 0});
       }
-      for (int k = 0; k < T.n_torsions; ++k) torsf.push_back(T.tors_frag[k]);
+      if (T.n_torsions < 0 || T.n_torsions >= T.n_frags)
+        return fail(PXD_ERR_CONF, "ligand %d: %d torsions for %d fragments", l, T.n_torsions,
+                    T.n_frags);
+      for (int k = 0; k < T.n_torsions; ++k) {
+        const int f = T.tors_frag[k];
+        if (f < 0 || f >= T.n_frags || T.frag_parent[f] < 0)
+          return fail(PXD_ERR_CONF, "ligand %d: torsion %d moves fragment %d, not a child fragment",
+                      l, k, f);
+        torsf.push_back(f);
+      }
       torsf.resize(I.frag_off + T.n_frags, 0);
       for (int i = 0; i < n; ++i) perm.push_back(T.perm[i]);
     } else {
@@ -1039,6 +1051,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
     I.n_rot = m.n_rotatable;
     natoms[l] = n;
     ntors[l] = I.n_tors;
+    nfrags[l] = I.n_frags;
   }
   Packer P;
   size_t o_info = P.add(info), o_type = P.add(type), o_q = P.add(q), o_lj = P.add(lj),
@@ -1092,6 +1105,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   b->h_ntors = new int[nlig];
   memcpy(b->h_natoms, natoms.data(), sizeof(int) * nlig);
   b->max_atoms = *std::max_element(natoms.begin(), natoms.end());
+  b->max_frags = *std::max_element(nfrags.begin(), nfrags.end());
   memcpy(b->h_ntors, ntors.data(), sizeof(int) * nlig);
   memcpy(b->type_used, used, sizeof(used));
   b->has_trees = trees ? 1 : 0;

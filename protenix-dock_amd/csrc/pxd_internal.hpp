@@ -209,6 +209,7 @@ struct pxd_ligand_batch {
   int* h_natoms;
   int* h_ntors;
   int max_atoms;              // largest ligand
+  int max_frags;              // most fragments of a ligand
   int has_trees;              // fragment trees were given (pose parameterisation)
   uint8_t type_used[PXD_NUM_VDW_TYPES];
 };

@@ -903,15 +903,30 @@ void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_sco
 
 enum { PH_INIT = 0, PH_MUTATE = 1, PH_REFINE = 2 };
 
+// Dynamic LDS of k_mc (floats), sized at launch for the batch's largest
+// ligand: pose arrays, ring geometry, apply_parameters scratch, then the
+// walker's current conformation (coordinates, then leaf torsions at cur_t).
+struct McDims {
+  int ring, apply, cur, cur_t, total;
+};
+
+__host__ __device__ constexpr McDims mc_dims(int ma, int mf, int nm, int nlv, int n_rings) {
+  auto r4 = [](int x) { return (x + 3) & ~3; };
+  McDims d{};
+  d.ring = pose_dims(ma, nm, nlv).total;
+  d.apply = d.ring + r4(6 * (n_rings + kMaxLigRings));
+  d.cur = d.apply + r4(apply_lds_floats(ma, mf));
+  d.cur_t = d.cur + r4(3 * ma);
+  d.total = d.cur_t + r4(nlv);
+  return d;
+}
+
 template <int VPL>
-__global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
-                                           const DevGrid* __restrict__ G, McArgs A, int vpl_lo) {
-  __shared__ __align__(16) float lds[PXD_POSE_LDS_FLOATS];
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) void k_mc(
+    DevReceptor R, DevLigands L, const DevGrid* __restrict__ G, McArgs A, int vpl_lo) {
   __shared__ float vpool[V_COUNT * kWave * VPL];
   __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ double s_terms[PXD_NUM_TERMS];
-  __shared__ float apl[PXD_APPLY_LDS_FLOATS];
-  __shared__ float curx[kMaxAtoms * 3 + kMaxLeaves];
   __shared__ int s_w;
   const int lane = lane_id();
   constexpr int NV = kWave * VPL;
@@ -925,20 +940,23 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
   c.R = &R;
   c.L = &L;
   c.G = G;
-  c.s = carve_pose_lds(lds);
+  const McDims md = mc_dims(A.max_atoms, A.max_frags, R.n_mov, nl, R.n_ring5 + R.n_ring6);
+  c.s = carve_pose_lds(dyn_lds, pose_dims(A.max_atoms, R.n_mov, nl));
   c.v = vpool;
   c.mode = 0;
   PscoreLds ps;
-  ps = carve_ring_lds(dyn_lds, R.n_ring5 + R.n_ring6);
+  ps = carve_ring_lds(dyn_lds + md.ring, R.n_ring5 + R.n_ring6);
   ps.terms = s_terms;
-  ApplyLds ap = carve_apply_lds(apl);
+  ApplyLds ap = carve_apply_lds(dyn_lds + md.apply, A.max_atoms, A.max_frags);
+  float* curx = dyn_lds + md.cur;
+  const int ct = md.cur_t - md.cur;  // leaf torsions follow the coordinates
   BoxF box;
   for (int d = 0; d < 3; ++d) {
     box.lo[d] = A.box_lo[d];
     box.hi[d] = A.box_hi[d];
   }
   float* x = c.vec(V_X);
-  float* curt = curx + kMaxAtoms * 3;
+  float* curt = curx + ct;
   for (;;) {
     if (lane == 0) s_w = atomicAdd(A.counter, 1);
     __syncthreads();
@@ -1075,7 +1093,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
       double psv = pscore_pose(R, L, li, sp, c.s, ps);
       if (phase == PH_INIT) {
         st.nevals = c.nevals;
-        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : ct + k - n3] = x[k];
         st.cur_e = st.best_e = e;
         st.cur_ps = st.best_ps = psv;
         for (int k = lane; k < n3; k += kWave) A.best_xyz[off + k] = x[k];
@@ -1105,7 +1123,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
         }
         bool inbox = all_in_box(x, n, box);
         if (acc && inbox) {
-          for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+          for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : ct + k - n3] = x[k];
           st.cur_e = e;
           st.cur_ps = psv;
           __syncthreads();
@@ -1117,7 +1135,7 @@ __global__ __launch_bounds__(64) void k_mc(DevReceptor R, DevLigands L,
         st.step++;
       } else {  // PH_REFINE (searcher.cpp:167-174)
         st.nevals += c.nevals;
-        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : kMaxAtoms * 3 + k - n3] = x[k];
+        for (int k = lane; k < c.n; k += kWave) curx[k < n3 ? k : ct + k - n3] = x[k];
         st.cur_e = e;
         st.cur_ps = psv;
         if (psv < st.best_ps) {
@@ -1145,7 +1163,10 @@ void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, cons
   // one launch per L-BFGS size class (the walker's ligand decides), each
   // with its own queue head: a.counter[0..kLbfgsClasses)
   McArgs c = a;
-  const size_t dyn = ring_lds_bytes(R);
+  c.max_atoms = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  c.max_frags = a.max_frags > 0 && a.max_frags <= kMaxFrags ? a.max_frags : kMaxFrags;
+  const size_t dyn = sizeof(float) * mc_dims(c.max_atoms, c.max_frags, R.n_mov, R.n_leaves,
+                                             R.n_ring5 + R.n_ring6).total;
   hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[2]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c,
                      kLbfgsClassVpl[1]);
   c.counter = a.counter + 1;

@@ -68,6 +68,13 @@ __host__ __device__ inline size_t lbfgs_slot_floats(int history) {
 constexpr int kLbfgsClasses = 3;
 constexpr int kLbfgsClassVpl[kLbfgsClasses] = {2, 4, kLbfgsVpl};
 
+// Occupancy target of the Monte-Carlo walker kernel (waves per SIMD); the
+// search launches this many walker slots per SIMD.
+#ifndef PXD_MC_WAVES
+#define PXD_MC_WAVES 2
+#endif
+constexpr int kMcWaves = PXD_MC_WAVES;
+
 struct PscoreArgs {
   int n_poses;
   const int* pose_lig;
@@ -115,6 +122,8 @@ struct McArgs {
   float* hist;
   int hist_m;      // history slots per wave in hist
   int step_end;
+  int max_atoms;   // largest ligand of the batch (sizes the dynamic LDS)
+  int max_frags;   // most fragments of a ligand of the batch
 };
 
 struct ClusterArgs {

@@ -113,15 +113,22 @@ struct ApplyLds {
 
 #define PXD_APPLY_LDS_FLOATS (pxd::kMaxAtoms * 6 + pxd::kMaxFrags * 16)
 
-__device__ inline ApplyLds carve_apply_lds(float* b) {
+// apply_parameters scratch for ligands of up to ma atoms and mf fragments
+__host__ __device__ constexpr int apply_lds_floats(int ma, int mf) { return 6 * ma + 16 * mf; }
+
+__device__ inline ApplyLds carve_apply_lds(float* b, int ma, int mf) {
   ApplyLds a;
   a.cp = b;
-  a.loc = a.cp + kMaxAtoms * 3;
-  a.M = a.loc + kMaxAtoms * 3;
-  a.O = a.M + kMaxFrags * 9;
-  a.ax = a.O + kMaxFrags * 3;
-  a.th = a.ax + kMaxFrags * 3;
+  a.loc = a.cp + ma * 3;
+  a.M = a.loc + ma * 3;
+  a.O = a.M + mf * 9;
+  a.ax = a.O + mf * 3;
+  a.th = a.ax + mf * 3;
   return a;
+}
+
+__device__ inline ApplyLds carve_apply_lds(float* b) {
+  return carve_apply_lds(b, kMaxAtoms, kMaxFrags);
 }
 
 __device__ inline void mat3mul(const float* a, const float* b, float* o) {

@@ -108,7 +108,7 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return pxd_set_error_(PXD_ERR_HIP, "mc_search: no current HIP device");
-  const int n_slots = std::min(W, std::max(1, ncu) * 4);
+  const int n_slots = std::min(W, std::max(1, ncu) * 4 * kMcWaves);
   const int hist_m = std::max(prm->relax.history, prm->full.history);
   const int n_launch = (prm->max_nsteps + prm->steps_per_launch - 1) / prm->steps_per_launch + 1;
   const size_t hist_floats = (size_t)n_slots * lbfgs_slot_floats(hist_m);
@@ -185,6 +185,8 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
     a.st = d_st;
     a.hist = d_hist;
     a.hist_m = hist_m;
+    a.max_atoms = b->max_atoms;
+    a.max_frags = b->max_frags;
     // capture: reset state and queue heads with fill kernels (no memset nodes),
     // then one k_mc launch per segment of steps, then clustering
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

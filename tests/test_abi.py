@@ -53,3 +53,17 @@ def test_tuning_defaults_and_validation():
     t = lib.get_tuning()
     assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu) == (16, 1, 4)
     lib.set_tuning()
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("tors_frag", 0, b"not a child fragment"),      # the root has no torsion
+    ("tors_frag", 99, b"not a child fragment"),
+    ("frag_bond", 10_000, b"bond atom out of range"),
+])
+def test_fragment_tree_is_validated_before_device_work(field, value, msg):
+    from pxdock_amd import synth
+    lig = synth.make_ligand(3, n_heavy=20, n_torsions=4, n_rings=1)
+    getattr(lig.tree, field)[-1] = value
+    with pytest.raises(lib.PxdError) as e:
+        lib.LigandBatch([lig])
+    assert msg in str(e.value).encode()
