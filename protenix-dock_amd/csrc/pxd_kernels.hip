@@ -562,7 +562,8 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
         auto fetch = [&](int h, float* ds, float* dy, double& rho) {  // h clamped to [0, num_old)
           h = min(max(h, 0), max(num_old - 1, 0));
-          const unsigned sl = ((unsigned)head + (unsigned)h) % (unsigned)m;
+          unsigned sl = (unsigned)head + (unsigned)h;  // head, h < m: one wrap at most
+          sl = sl >= (unsigned)m ? sl - (unsigned)m : sl;
           const size_t o = (size_t)sl * NV + lane;
 #pragma unroll
           for (int k = 0; k < VPL; ++k) {
