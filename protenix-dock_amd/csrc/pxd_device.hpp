@@ -21,12 +21,15 @@ namespace pxd {
 // which is read back as a uniform value.  GFX9 DPP controls: quad_perm 0xB1
 // (xor 1) and 0x4E (xor 2), row_half_mirror 0x141, row_mirror 0x140,
 // row_bcast:15 0x142 (rows 1 and 3), row_bcast:31 0x143 (rows 2 and 3).
-// With all rows enabled every lane is written and the old value is unused
-// (mov_dpp, no initialising move); with rows masked off the disabled rows
-// keep old = 0 (update_dpp), so the following add leaves them unchanged.
+// With all rows enabled every lane is written from a valid source lane, so
+// bound_ctrl changes nothing; setting it lets the compiler fold the lane move
+// into the add (one v_add_f32_dpp per level instead of a move and an add).
+// With rows masked off the disabled rows keep old = 0 (update_dpp), so the
+// following add leaves them unchanged.
 template <int CTRL, int ROWS>
 __device__ inline float dpp_f(float v) {
-  if (ROWS == 0xF) return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  if (ROWS == 0xF)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
   return __int_as_float(
       __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }

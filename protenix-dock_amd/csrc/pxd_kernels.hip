@@ -560,16 +560,20 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         double br[kPre];
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
+        // A slot holds the class's full width (VPL elements per lane), so
+        // every element of a slot is read from one wave-uniform base with
+        // constant offsets; elements past the pose's nk are stale values
+        // that the arithmetic masks out by selects (never by multiplying).
         auto fetch = [&](int h, float* ds, float* dy, double& rho) {  // h clamped to [0, num_old)
           h = min(max(h, 0), max(num_old - 1, 0));
           unsigned sl = (unsigned)head + (unsigned)h;  // head, h < m: one wrap at most
           sl = sl >= (unsigned)m ? sl - (unsigned)m : sl;
-          const size_t o = (size_t)sl * NV + lane;
+          const float* ps = hs + (size_t)sl * NV;
+          const float* py = hy + (size_t)sl * NV;
 #pragma unroll
           for (int k = 0; k < VPL; ++k) {
-            const int kk = min(k, nk1);
-            ds[k] = hs[o + kWave * kk];
-            dy[k] = hy[o + kWave * kk];
+            ds[k] = ps[lane + kWave * k];
+            dy[k] = py[lane + kWave * k];
           }
           rho = hr[sl];
         };
