@@ -488,7 +488,9 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
                             bool* converged_out) {
   const int lane = lane_id();
   const int m = P.history;
-  const int NV = kWave * c.nk;  // history entries are packed at the vector length in use
+  // history entries are stored at the class's full width (VPL elements per
+  // lane; m entries fill the caller's history*64*VPL floats of hy and of hs)
+  constexpr int NV = kWave * VPL;
   float* x = c.vec(V_X);
   float* g = c.vec(V_G);
   float* gp = c.vec(V_GP);
@@ -535,10 +537,13 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             num_old++;
           }
           H = ys / vdot<VPL>(c.nk, y, y);
+          // the slot's full class width is written, zeros past the pose's
+          // nk elements, so the two-loop needs no masks
 #pragma unroll
-          for (int k = 0; k < VPL; ++k) if (k < c.nk) {
-            hy[(size_t)slot * NV + lane + kWave * k] = y[lane + kWave * k];
-            hs[(size_t)slot * NV + lane + kWave * k] = sv[lane + kWave * k];
+          for (int k = 0; k < VPL; ++k) {
+            const bool in = k < c.nk;
+            hy[(size_t)slot * NV + lane + kWave * k] = in ? y[lane + kWave * k] : 0.f;
+            hs[(size_t)slot * NV + lane + kWave * k] = in ? sv[lane + kWave * k] : 0.f;
           }
           hr[slot] = 1.0 / ys;  // every lane stores (and later reads) the same value
           __syncthreads();
@@ -549,21 +554,21 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         // history entries are loaded kPre steps ahead of their use into a
         // ring of register buffers (statically indexed by unrolling kPre
         // steps).  Every load is unconditional — indices are clamped, and a
-        // step past the last entry or an element past n is masked out of the
-        // arithmetic instead of branched around — so the compiler can count
-        // the loads in flight and only waits for the one entry it needs.
+        // step past the last entry is masked out of the arithmetic instead of
+        // branched around — so the compiler can count the loads in flight and
+        // only waits for the one entry it needs.
         // The arithmetic does not depend on kPre or VPL: every size class
         // gives the same result for the same pose.
         constexpr int kPre = VPL <= 2 ? 6 : (VPL <= 4 ? 4 : 2);
-        const int nk1 = c.nk - 1;
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
         double br[kPre];
 #pragma unroll
         for (int k = 0; k < VPL; ++k) qr[k] = k < c.nk ? -g[lane + kWave * k] : 0.f;
         // A slot holds the class's full width (VPL elements per lane), so
         // every element of a slot is read from one wave-uniform base with
-        // constant offsets; elements past the pose's nk are stale values
-        // that the arithmetic masks out by selects (never by multiplying).
+        // constant offsets.  Elements past the pose's nk are zero in every
+        // entry of this pose (and in q), so they add exact zeros: the
+        // arithmetic needs no masks and is the same in every size class.
         auto fetch = [&](int h, float* ds, float* dy, double& rho) {  // h clamped to [0, num_old)
           h = min(max(h, 0), max(num_old - 1, 0));
           unsigned sl = (unsigned)head + (unsigned)h;  // head, h < m: one wrap at most
@@ -586,14 +591,13 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             const int h = num_old - 1 - (grp * kPre + j);
             float acc = 0.f;
 #pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              acc += k <= nk1 ? bs[j][k] * qr[k] : 0.f;
+            for (int k = 0; k < VPL; ++k) acc += bs[j][k] * qr[k];
             double al = (double)wave_sum(acc) * br[j];
             al = h >= 0 ? al : 0.0;
             s_al[h >= 0 ? h : kMaxHistory] = al;  // every lane stores the same value
             const float alf = (float)al;
 #pragma unroll
-            for (int k = 0; k < VPL; ++k) qr[k] -= (k <= nk1 ? alf : 0.f) * by[j][k];
+            for (int k = 0; k < VPL; ++k) qr[k] -= alf * by[j][k];
             fetch(h - kPre, bs[j], by[j], br[j]);
           }
         }
@@ -613,13 +617,12 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             const int h = grp * kPre + j;
             float acc = 0.f;
 #pragma unroll
-            for (int k = 0; k < VPL; ++k)
-              acc += k <= nk1 ? by[j][k] * qr[k] : 0.f;
+            for (int k = 0; k < VPL; ++k) acc += by[j][k] * qr[k];
             const double be = (double)wave_sum(acc) * br[j];
             float cf = (float)(ba[j] - be);
             cf = h < num_old ? cf : 0.f;
 #pragma unroll
-            for (int k = 0; k < VPL; ++k) qr[k] += bs[j][k] * (k <= nk1 ? cf : 0.f);
+            for (int k = 0; k < VPL; ++k) qr[k] += bs[j][k] * cf;
             fetch(h + kPre, bs[j], by[j], br[j]);
             ba[j] = s_al[max(min(h + kPre, num_old - 1), 0)];
           }
