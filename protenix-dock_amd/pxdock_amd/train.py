@@ -62,9 +62,13 @@ def _fn():
                 s, dx, dp = lib.pscore_grad(receptor, poses, sp)
             finally:
                 poses.xyz = saved
-            atom_pose = torch.from_numpy(np.repeat(np.arange(poses.n_poses),
-                                                   poses.batch.natoms[poses.pose_lig])).to(
-                xyz.device)
+            atom_pose = getattr(poses, "_atom_pose", None)
+            if atom_pose is None or atom_pose.device != xyz.device:
+                # pose index of every atom of the CSR layout, built once per PoseSet
+                atom_pose = torch.from_numpy(np.repeat(np.arange(poses.n_poses),
+                                                       poses.batch.natoms[poses.pose_lig])).to(
+                    xyz.device)
+                poses._atom_pose = atom_pose
             ctx.save_for_backward(dx, dp, atom_pose)
             return s
 
@@ -77,7 +81,10 @@ def _fn():
                 per_atom = gs.to(dx.dtype)[atom_pose]
                 g_xyz = (dx.view(-1, 3) * per_atom[:, None]).view(-1)
             if ctx.needs_input_grad[1]:
-                g_theta = gs.to(torch.float64) @ dp
+                # g^T dS/dtheta as a column sum over the poses: a 1 x n_poses
+                # by n_poses x N_PARAMS product handed to a GEMM runs its long
+                # inner dimension in two tiles, ~100x slower than this
+                g_theta = (dp * gs.to(torch.float64)[:, None]).sum(0)
             return g_xyz, g_theta, None, None, None
 
     return PoseScore
