@@ -260,8 +260,9 @@ __device__ inline float dihedral(const float* pi, const float* pj, const float* 
 // Coulomb + LJ of one pair (pair.h:25-60) with r = x_j - x_i.  qq carries
 // q_i q_j scale, e4 carries sqrt(eps_i eps_j) * 4 * scale.  Returns the energy
 // and writes de_dr_r (the gradient on i is -r * de_dr_r, on j +r * de_dr_r).
-__device__ inline float pair_energy(float rx, float ry, float rz, float qq, float sig,
-                                    float e4, float* de_dr_r) {
+// qk = qq * kCoulomb given by the caller (pair_energy below forms it)
+__device__ inline float pair_energy_qk(float rx, float ry, float rz, float qk, float sig,
+                                       float e4, float* de_dr_r) {
   float d2 = rx * rx + ry * ry + rz * rz;
   // 1/(d + eps) from one hardware reciprocal square root (1 ulp); at d = 0
   // it saturates at 1/eps = 1e20 as safe_divide does (math.h:176-178).
@@ -269,7 +270,6 @@ __device__ inline float pair_energy(float rx, float ry, float rz, float qq, floa
   // rounding for every d > 1e-6 A.
   float rd = fminf(__builtin_amdgcn_rsqf(d2), 1e20f);
   float rd2 = rd * rd;
-  float qk = qq * kCoulomb;
   float ec = qk * rd;
   float dc = -qk * rd2 * rd;
   float u = sig * rd;
@@ -279,6 +279,11 @@ __device__ inline float pair_energy(float rx, float ry, float rz, float qq, floa
   float dv = (-12.f * u12 + 6.f * u6) * rd2 * e4;
   *de_dr_r = dc + dv;
   return ec + (u12 - u6) * e4;
+}
+
+__device__ inline float pair_energy(float rx, float ry, float rz, float qq, float sig,
+                                    float e4, float* de_dr_r) {
+  return pair_energy_qk(rx, ry, rz, qq * kCoulomb, sig, e4, de_dr_r);
 }
 
 // ---------------------------------------------------------------- per-pose LDS
@@ -686,10 +691,16 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
     }
     stage_gather(s, gw, g0, g1);
   }
+  // pairs by partner masks (ligand_pairs_ring / _masked): the gradient
+  // stays in registers and is stored at the end of inter, so the region the
+  // stage overlays is left to the ring's partner records
+  if (li.pmode) {
+    __syncthreads();
+    return e;
+  }
   // the stage is free again: clear the gradient array it overlays
   for (int k = lane; k < li.n_atoms * 3; k += kWave) s.grad[k] = 0.f;
   __syncthreads();
-  if (li.pmode) return e;  // pairs by partner masks (ligand_pairs_masked)
   for (int p = lane; p < li.n_pairs; p += kWave) {
     uint32_t w = L.pairs[li.pair_off + p];
     int i = w & 255, j = (w >> 8) & 255;
@@ -792,7 +803,10 @@ __device__ inline double ligand_pairs_masked(const DevLigands& L, const LigInfo&
 // partner's reaction goes into a travelling accumulator t, which holds atom
 // a + k's sum in step k and moves one lane down the ring (ds_bpermute) after
 // each step; at the end it is sent home.  The partner bits come from the
-// owned atom's masks rotated so that bit k is partner a + k.
+// owned atom's masks rotated so that bit k is partner a + k.  Partner data
+// come from 32-byte records (x, y, z, q, sigma / 2, sqrt(eps)) written twice,
+// at j and j + n, into the gradient region (free here: this mode keeps the
+// gradient in registers), so partner a + k is one pointer step away.
 __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& li,
                                            const PoseLds& s, float* g0) {
   const int lane = lane_id(), n = li.n_atoms;
@@ -810,12 +824,23 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   }
   const float x0 = s.xyz[3 * a], y0 = s.xyz[3 * a + 1], z0 = s.xyz[3 * a + 2], q0 = s.q[a];
   const float2 l0 = s.lj[a];
+  static_assert(2 * kWave * 8 <= 4 * kStageSlots, "2n records of 8 floats fit the gradient region");
+  float4* rec = reinterpret_cast<float4*>(s.grad);  // 2n records of 2 float4 (n <= 64)
+  if (act) {
+    const float4 r0 = make_float4(x0, y0, z0, q0), r1 = make_float4(0.5f * l0.x, l0.y, 0.f, 0.f);
+    rec[2 * a] = r0;
+    rec[2 * a + 1] = r1;
+    rec[2 * (a + n)] = r0;
+    rec[2 * (a + n) + 1] = r1;
+  }
+  __syncthreads();
+  // own factors of the pair parameters: q_a * kCoulomb, sigma_a / 2, 4 sqrt(eps_a)
+  const float q0k = q0 * kCoulomb, sh0 = 0.5f * l0.x, e40 = 4.f * l0.y;
   const int nxt = act ? (a + 1 == n ? 0 : a + 1) : lane;  // ring successor (source of t)
   const int K = n / 2;
   float t[3] = {0.f, 0.f, 0.f}, es = 0.f;
-  int b = a;
-  for (int k = 1; k <= K; ++k) {
-    b = b + 1 == n ? 0 : b + 1;
+  const float4* pr = rec + 2 * (a + 1);  // partner a + k's record in step k
+  for (int k = 1; k <= K; ++k, pr += 2) {
     ra >>= 1;
     rf >>= 1;
     const bool ia = ra & 1ull, i4 = rf & 1ull;
@@ -823,11 +848,11 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
     if (live) {
       const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
       const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
-      const float2 lb = s.lj[b];
-      const float rx = s.xyz[3 * b] - x0, ry = s.xyz[3 * b + 1] - y0, rz = s.xyz[3 * b + 2] - z0;
+      const float4 pb = pr[0];
+      const float2 lb = *reinterpret_cast<const float2*>(pr + 1);
+      const float rx = pb.x - x0, ry = pb.y - y0, rz = pb.z - z0;
       float de;
-      es += pair_energy(rx, ry, rz, q0 * s.q[b] * cs, (l0.x + lb.x) * 0.5f, l0.y * lb.y * 4.f * vs,
-                        &de);
+      es += pair_energy_qk(rx, ry, rz, q0k * pb.w * cs, sh0 + lb.x, e40 * lb.y * vs, &de);
       g0[0] -= rx * de;
       g0[1] -= ry * de;
       g0[2] -= rz * de;
@@ -1004,8 +1029,15 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
       tl = 0.f;
     }
   }
-  if (a0 < n) lig_grad_add(s, a0, g0, 1.f);
-  if (a1 < n) lig_grad_add(s, a1, g1, 1.f);
+  if (li.pmode) {  // nothing else wrote s.grad: store (the region held scratch)
+    for (int d = 0; d < 3; ++d) {
+      if (a0 < n) s.grad[3 * a0 + d] = g0[d];
+      if (a1 < n) s.grad[3 * a1 + d] = g1[d];
+    }
+  } else {
+    if (a0 < n) lig_grad_add(s, a0, g0, 1.f);
+    if (a1 < n) lig_grad_add(s, a1, g1, 1.f);
+  }
   return e;
 }
 
