@@ -691,7 +691,7 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
     }
     stage_gather(s, gw, g0, g1);
   }
-  // pairs by partner masks (ligand_pairs_ring / _masked): the gradient
+  // pairs by partner masks (ligand_pairs_ring / _ring2): the gradient
   // stays in registers and is stored at the end of inter, so the region the
   // stage overlays is left to the ring's partner records
   if (li.pmode) {
@@ -718,85 +718,9 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
   return e;
 }
 
-// Ligand non-bonded pairs (interaction.cpp:127-168) by partner masks: lane
-// owns atoms lane and lane + 64 and visits every partner b in order (LDS
-// broadcast reads, no atomics); a pair listed as all-pairs and/or 1-4
-// contributes with the sum of its scales.  Each pair is met from both ends,
-// each end adding half its energy and its own gradient (g0 / g1, registers).
-__device__ inline double ligand_pairs_masked(const DevLigands& L, const LigInfo& li,
-                                             const PoseLds& s, float* g0, float* g1) {
-  const int lane = lane_id(), n = li.n_atoms;
-  const bool two = n > kWave;
-  const int a0 = lane, a1 = lane + kWave;
-  uint4 m0a = make_uint4(0, 0, 0, 0), m0f = m0a, m1a = m0a, m1f = m0a;
-  float x0[3] = {0.f, 0.f, 0.f}, x1[3] = {0.f, 0.f, 0.f}, q0 = 0.f, q1 = 0.f;
-  float2 l0 = make_float2(0.f, 0.f), l1 = l0;
-  const uint4* pm = L.pmask + 2ll * li.atom_off;
-  if (a0 < n) {
-    m0a = pm[2 * a0];
-    m0f = pm[2 * a0 + 1];
-    x0[0] = s.xyz[3 * a0];
-    x0[1] = s.xyz[3 * a0 + 1];
-    x0[2] = s.xyz[3 * a0 + 2];
-    q0 = s.q[a0];
-    l0 = s.lj[a0];
-  }
-  if (a1 < n) {
-    m1a = pm[2 * a1];
-    m1f = pm[2 * a1 + 1];
-    x1[0] = s.xyz[3 * a1];
-    x1[1] = s.xyz[3 * a1 + 1];
-    x1[2] = s.xyz[3 * a1 + 2];
-    q1 = s.q[a1];
-    l1 = s.lj[a1];
-  }
-  // partners in blocks of 32 (one mask word per block, read once); the pair
-  // energies of a block are summed in FP32 and added to the FP64 total once
-  double e = 0.0;
-  for (int w = 0; 32 * w < n; ++w) {
-    const uint32_t wa0 = w == 0 ? m0a.x : w == 1 ? m0a.y : w == 2 ? m0a.z : m0a.w;
-    const uint32_t wf0 = w == 0 ? m0f.x : w == 1 ? m0f.y : w == 2 ? m0f.z : m0f.w;
-    const uint32_t wa1 = w == 0 ? m1a.x : w == 1 ? m1a.y : w == 2 ? m1a.z : m1a.w;
-    const uint32_t wf1 = w == 0 ? m1f.x : w == 1 ? m1f.y : w == 2 ? m1f.z : m1f.w;
-    const int bend = min(32, n - 32 * w);
-    float es = 0.f;
-    for (int t = 0; t < bend; ++t) {
-      const int b = 32 * w + t;
-      const float xb = s.xyz[3 * b], yb = s.xyz[3 * b + 1], zb = s.xyz[3 * b + 2], qb = s.q[b];
-      const float2 lb = s.lj[b];
-      {
-        const bool ia = (wa0 >> t) & 1u, i4 = (wf0 >> t) & 1u;
-        if (ia || i4) {
-          float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
-          float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
-          float rx = xb - x0[0], ry = yb - x0[1], rz = zb - x0[2], de;
-          es += pair_energy(rx, ry, rz, q0 * qb * cs, (l0.x + lb.x) * 0.5f, l0.y * lb.y * 4.f * vs,
-                            &de);
-          g0[0] -= rx * de;
-          g0[1] -= ry * de;
-          g0[2] -= rz * de;
-        }
-      }
-      if (two) {
-        const bool ia = (wa1 >> t) & 1u, i4 = (wf1 >> t) & 1u;
-        if (ia || i4) {
-          float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
-          float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
-          float rx = xb - x1[0], ry = yb - x1[1], rz = zb - x1[2], de;
-          es += pair_energy(rx, ry, rz, q1 * qb * cs, (l1.x + lb.x) * 0.5f, l1.y * lb.y * 4.f * vs,
-                            &de);
-          g1[0] -= rx * de;
-          g1[1] -= ry * de;
-          g1[2] -= rz * de;
-        }
-      }
-    }
-    e += 0.5 * (double)es;
-  }
-  return e;
-}
-
-// Ligand non-bonded pairs for ligands of at most 64 atoms, each pair met
+// Ligand non-bonded pairs (interaction.cpp:127-168) by partner masks (no
+// atomics); a pair listed as all-pairs and/or 1-4 contributes with the sum
+// of its scales.  Ligands of at most 64 atoms: each pair met
 // once: lane a owns atom a; in step k = 1 .. n/2 it takes the pair
 // (a, a + k mod n) — every unordered pair once, the last step of an even n
 // only from its lower half.  Lane a keeps its own gradient in g0; the
@@ -875,6 +799,128 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
       g0[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * home, __float_as_int(t[d])));
   }
   return (double)es;
+}
+
+// Ligand non-bonded pairs for ligands of 65 .. 128 atoms, each pair met once
+// on the same ring schedule: lane l owns atoms l (slot 0) and l + 64 (slot
+// 1, when < n), owner index i = l + 64 * slot; in step k = 1 .. n/2 owner i
+// takes the pair (i, i + k mod n).  Each slot keeps its own gradient in
+// registers and carries a travelling accumulator for its partner; after a
+// step every accumulator moves from owner i to owner i - 1 mod n (slot 0's
+// lane 63 takes slot 1's lane 0, slot 1's last owner takes owner 0), and at
+// the end atom j's sum is fetched from owner j - n/2 mod n.
+__device__ inline uint64_t ring_bits(uint4 m, int i, int n) {
+  // bits 1 .. 64 of the n-bit mask m rotated right by i (bit k <- bit (i + k) mod n)
+  typedef unsigned __int128 u128;
+  const u128 full = n == 128 ? ~(u128)0 : (((u128)1 << n) - 1);
+  const u128 w = (((u128)m.w << 96) | ((u128)m.z << 64) | ((u128)m.y << 32) | (u128)m.x) & full;
+  const u128 r = i ? ((w >> i) | (w << (n - i))) & full : w;
+  return (uint64_t)(r >> 1);
+}
+
+__device__ inline double ligand_pairs_ring2(const DevLigands& L, const LigInfo& li,
+                                            const PoseLds& s, float* g0, float* g1) {
+  const int lane = lane_id(), n = li.n_atoms;  // 64 < n <= 128
+  const int i0 = lane, i1 = lane + kWave;
+  const bool act1 = i1 < n;
+  const int j1 = act1 ? i1 : i0;  // inactive slot 1 reads atom i0 (never live)
+  const uint4* pm = L.pmask + 2ll * li.atom_off;
+  uint64_t a0 = ring_bits(pm[2 * i0], i0, n), f0 = ring_bits(pm[2 * i0 + 1], i0, n);
+  uint64_t a1 = 0, f1 = 0;
+  if (act1) {
+    a1 = ring_bits(pm[2 * i1], i1, n);
+    f1 = ring_bits(pm[2 * i1 + 1], i1, n);
+  }
+  const float x0 = s.xyz[3 * i0], y0 = s.xyz[3 * i0 + 1], z0 = s.xyz[3 * i0 + 2];
+  const float x1 = s.xyz[3 * j1], y1 = s.xyz[3 * j1 + 1], z1 = s.xyz[3 * j1 + 2];
+  const float2 l0 = s.lj[i0], l1 = s.lj[j1];
+  // own factors of the pair parameters: q * kCoulomb, sigma / 2, 4 sqrt(eps)
+  const float q0k = s.q[i0] * kCoulomb, sh0 = 0.5f * l0.x, e40 = 4.f * l0.y;
+  const float q1k = s.q[j1] * kCoulomb, sh1 = 0.5f * l1.x, e41 = 4.f * l1.y;
+  const int K = n / 2;
+  const int src = (lane + 1) & (kWave - 1);  // owner i + 1 of slot 0 (lane 63: slot 1's lane 0)
+  const bool last1 = i1 + 1 == n;            // slot 1's last owner: its successor is owner 0
+  float t0[3] = {0.f, 0.f, 0.f}, t1[3] = {0.f, 0.f, 0.f}, es = 0.f;
+  double e = 0.0;
+  int b0 = i0, b1 = j1;
+  for (int k = 1; k <= K; ++k) {
+    b0 = b0 + 1 == n ? 0 : b0 + 1;
+    b1 = b1 + 1 == n ? 0 : b1 + 1;
+    const bool even_last = 2 * k == n;  // the pair (i, i + n/2) only from i < n/2
+    {
+      const bool ia = a0 & 1ull, i4 = f0 & 1ull;
+      if ((ia || i4) && !(even_last && i0 >= k)) {
+        const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+        const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+        const float2 lb = s.lj[b0];
+        const float rx = s.xyz[3 * b0] - x0, ry = s.xyz[3 * b0 + 1] - y0,
+                    rz = s.xyz[3 * b0 + 2] - z0;
+        float de;
+        es += pair_energy_qk(rx, ry, rz, q0k * s.q[b0] * cs, fmaf(lb.x, 0.5f, sh0),
+                             e40 * lb.y * vs, &de);
+        g0[0] -= rx * de;
+        g0[1] -= ry * de;
+        g0[2] -= rz * de;
+        t0[0] += rx * de;
+        t0[1] += ry * de;
+        t0[2] += rz * de;
+      }
+    }
+    {
+      const bool ia = a1 & 1ull, i4 = f1 & 1ull;  // zero masks when slot 1 is inactive
+      if ((ia || i4) && !(even_last && i1 >= k)) {
+        const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
+        const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+        const float2 lb = s.lj[b1];
+        const float rx = s.xyz[3 * b1] - x1, ry = s.xyz[3 * b1 + 1] - y1,
+                    rz = s.xyz[3 * b1 + 2] - z1;
+        float de;
+        es += pair_energy_qk(rx, ry, rz, q1k * s.q[b1] * cs, fmaf(lb.x, 0.5f, sh1),
+                             e41 * lb.y * vs, &de);
+        g1[0] -= rx * de;
+        g1[1] -= ry * de;
+        g1[2] -= rz * de;
+        t1[0] += rx * de;
+        t1[1] += ry * de;
+        t1[2] += rz * de;
+      }
+    }
+    a0 >>= 1;
+    f0 >>= 1;
+    a1 >>= 1;
+    f1 >>= 1;
+    if ((k & 31) == 0) {  // FP32 pair energies per block of 32 steps, then FP64
+      e += (double)es;
+      es = 0.f;
+    }
+    if (k < K) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const float h0 = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(t0[d])));
+        const float h1 = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(t1[d])));
+        const float z = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t0[d]), 0));
+        t0[d] = lane == kWave - 1 ? h1 : h0;
+        t1[d] = last1 ? z : h1;
+      }
+    }
+  }
+  e += (double)es;
+  // atom j's sum sits at owner (j - K) mod n
+#pragma unroll
+  for (int slot = 0; slot < 2; ++slot) {
+    const int j = slot ? j1 : i0;
+    int o = j - K;
+    if (o < 0) o += n;
+    const int sl = o & (kWave - 1);
+    float* g = slot ? g1 : g0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const float h0 = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * sl, __float_as_int(t0[d])));
+      const float h1 = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * sl, __float_as_int(t1[d])));
+      if (slot == 0 || act1) g[d] += o < kWave ? h0 : h1;
+    }
+  }
+  return e;
 }
 
 // Trilinear value + stored-gradient interpolation (grid.cpp:273-303).
@@ -1087,7 +1133,7 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
     if (li.n_atoms <= kWave)
       e += ligand_pairs_ring(L, li, s, g0);
     else
-      e += ligand_pairs_masked(L, li, s, g0, g1);
+      e += ligand_pairs_ring2(L, li, s, g0, g1);
   }
   e += inter(R, G, li, s, g0, g1);
   __syncthreads();

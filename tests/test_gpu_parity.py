@@ -165,11 +165,14 @@ def test_gpu_score_grad_matches_oracle_with_maps(world):
 
 def test_gpu_pair_schedules_at_size_boundaries(world):
     """Ligands of 3, 5, 33, 63, 64 atoms take the ring pair schedule (each
-    pair once, reactions in a travelling accumulator); 65 and 66 atoms take
-    the two-atoms-per-lane sweep.  Energies and gradients against the oracle
-    on the same maps, poses near the pocket (many close pairs)."""
+    pair once, reactions in a travelling accumulator); 65, 66, 96, 100, 127
+    and 128 atoms take the two-slot ring (atoms lane and lane + 64, the
+    accumulators moving across the slot boundary and wrapping at n).
+    Energies and gradients against the oracle on the same maps, poses near
+    the pocket (many close pairs)."""
     picks = [(5001, 2, 3), (5013, 3, 5), (5007, 28, 33), (5004, 54, 63), (5018, 54, 64),
-             (5000, 54, 65), (5002, 54, 66)]
+             (5000, 54, 65), (5002, 54, 66), (5005, 75, 96), (5008, 81, 100),
+             (5005, 102, 127), (5002, 103, 128)]
     ligs = []
     for seed, nh, n in picks:
         m = synth.make_ligand(seed, n_heavy=nh, n_torsions=min(3, max(0, nh - 3)), n_rings=0)
