@@ -1044,12 +1044,14 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
     }
   }
   // movable receptor atoms: explicit pairs; as in receptor_intra, the slots'
-  // gradients enter through jac . g summed per leaf (one wave sum per leaf)
-  float tl = 0.f;
+  // gradients enter through jac . g summed per leaf (one wave sum per leaf).
+  // The movable atom's factors of the pair parameters (q * kCoulomb,
+  // sigma / 2, 4 sqrt(eps)) are formed once per atom for the wave; the pair
+  // energies are summed in FP32 per lane, then added to the FP64 total.
+  float tl = 0.f, em = 0.f;
   for (int m = 0; m < R.n_mov; ++m) {
     const float4 mp = s.movp[m];
-    const float qm = mp.x;
-    const float2 lm = make_float2(mp.y, mp.z);
+    const float qmk = mp.x * kCoulomb, smh = 0.5f * mp.y, em4 = 4.f * mp.z;
     float px = s.mov[3 * m], py = s.mov[3 * m + 1], pz = s.mov[3 * m + 2];
     float gx = 0.f, gy = 0.f, gz = 0.f;
     for (int h = 0; h < 2; ++h) {
@@ -1059,7 +1061,7 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
       float rx = px - s.xyz[3 * a], ry = py - s.xyz[3 * a + 1], rz = pz - s.xyz[3 * a + 2];
       float2 la = s.lj[a];
       float de;
-      e += pair_energy(rx, ry, rz, s.q[a] * qm, (la.x + lm.x) * 0.5f, la.y * lm.y * 4.f, &de);
+      em += pair_energy_qk(rx, ry, rz, s.q[a] * qmk, fmaf(la.x, 0.5f, smh), la.y * em4, &de);
       gg[0] -= rx * de;
       gg[1] -= ry * de;
       gg[2] -= rz * de;
@@ -1075,6 +1077,7 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
       tl = 0.f;
     }
   }
+  e += (double)em;
   if (li.pmode) {  // nothing else wrote s.grad: store (the region held scratch)
     for (int d = 0; d < 3; ++d) {
       if (a0 < n) s.grad[3 * a0 + d] = g0[d];
