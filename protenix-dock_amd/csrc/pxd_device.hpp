@@ -764,14 +764,14 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   const int K = n / 2;
   float t[3] = {0.f, 0.f, 0.f}, es = 0.f;
   const float4* pr = rec + 2 * (a + 1);  // partner a + k's record in step k
-  for (int k = 1; k <= K; ++k, pr += 2) {
-    ra >>= 1;
-    rf >>= 1;
-    const bool ia = ra & 1ull, i4 = rf & 1ull;
-    const bool live = act && (ia || i4) && !(2 * k == n && a >= k);
+  // bits 1 .. 32 of the rotated masks (K <= 32 steps), bit 0 = step 1
+  uint32_t ba = (uint32_t)(ra >> 1), bf = (uint32_t)(rf >> 1);
+  for (int k = 1; k <= K; ++k, pr += 2, ba >>= 1, bf >>= 1) {
+    const uint32_t ia = ba & 1u, i4 = bf & 1u;
+    const bool live = act && (ia | i4) && !(2 * k == n && a >= k);
     if (live) {
-      const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
-      const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
+      const float fa = (float)ia, f4 = (float)i4;  // scales: 1, the 1-4 scale, or their sum
+      const float cs = fmaf(f4, kScale14Coul, fa), vs = fmaf(f4, kScale14Vdw, fa);
       const float4 pb = pr[0];
       const float2 lb = *reinterpret_cast<const float2*>(pr + 1);
       const float rx = pb.x - x0, ry = pb.y - y0, rz = pb.z - z0;
