@@ -454,6 +454,9 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
       float dp[3], hp[3], gd[3], gh[3], ga[3];
       lpos(dh.x, dp);
       lpos(dh.y, hp);
+      // the donor's and hydrogen's gradients summed over the receptor
+      // acceptors in registers, added to the shared array once
+      float sd[3] = {0.f, 0.f, 0.f}, sh[3] = {0.f, 0.f, 0.f};
       for (int b = 0; b < R.n_acc_c + R.n_acc_n; ++b) {
         bool acc_c = b < R.n_acc_c;
         int j = acc_c ? R.acc_c[b] : R.acc_n[b - R.n_acc_c];
@@ -461,10 +464,14 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         float ap[3];
         rpos(j, ap);
         e8[sub] -= (double)hb(dp, hp, ap, sp.hbond_inter[sub], gd, gh, ga);
-        float f = -w * (float)sp.hbond_sub_coef[sub];
-        gadd(dh.x, gd, f);
-        gadd(dh.y, gh, f);
+        const float f = -w * (float)sp.hbond_sub_coef[sub];
+        for (int k = 0; k < 3; ++k) {
+          sd[k] += gd[k] * f;
+          sh[k] += gh[k] * f;
+        }
       }
+      gadd(dh.x, sd, 1.f);
+      gadd(dh.y, sh, 1.f);
     }
     int na = li.n_accc + li.n_accn;
     for (int a = lane; a < na; a += kWave) {
@@ -472,6 +479,7 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
       int i = L.sets[(acc_c ? li.accc_off : li.accn_off) + (acc_c ? a : a - li.n_accc)];
       float ap[3], gd[3], gh[3], ga[3];
       lpos(i, ap);
+      float sa[3] = {0.f, 0.f, 0.f};  // the acceptor's gradient over the receptor donors
       for (int b = 0; b < R.n_don_c + R.n_don_n; ++b) {
         bool charged = b < R.n_don_c;
         int2 dh = charged ? R.don_c[b] : R.don_n[b - R.n_don_c];
@@ -480,8 +488,10 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         rpos(dh.x, dp);
         rpos(dh.y, hp);
         e8[sub] -= (double)hb(dp, hp, ap, sp.hbond_inter[sub], gd, gh, ga);
-        gadd(i, ga, -w * (float)sp.hbond_sub_coef[sub]);
+        const float f = -w * (float)sp.hbond_sub_coef[sub];
+        for (int k = 0; k < 3; ++k) sa[k] += ga[k] * f;
       }
+      gadd(i, sa, 1.f);
     }
     double e = 0.0;
     for (int k = 0; k < 8; ++k) e += sp.hbond_sub_coef[k] * e8[k];
