@@ -296,15 +296,17 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
       float ang = acosf(fminf(fmaxf(cs, -1.f), 1.f)) / 3.141592653589793f * 180.f;
       return dr * rev_linear_c(ang, (float)s6[5], (float)s6[2], (float)s6[3]);
     };
-    // ligand donors x receptor acceptors: hldcc, hldcn, hldnc, hldnn
+    // ligand donors x receptor acceptors: hldcc, hldcn, hldnc, hldnn.  Lanes
+    // run over the receptor acceptors (hundreds), the ligand's few donors in
+    // turn; the per-lane sums are reduced with the other terms.
     int nd = li.n_donc + li.n_donn;
-    for (int a = lane; a < nd; a += kWave) {
+    for (int a = 0; a < nd; ++a) {
       bool charged = a < li.n_donc;
       int2 dh = L.dons[(charged ? li.donc_off : li.donn_off) + (charged ? a : a - li.n_donc)];
       float dp[3], hp[3];
       lpos(dh.x, dp);
       lpos(dh.y, hp);
-      for (int b = 0; b < R.n_acc_c + R.n_acc_n; ++b) {
+      for (int b = lane; b < R.n_acc_c + R.n_acc_n; b += kWave) {
         bool acc_c = b < R.n_acc_c;
         int j = acc_c ? R.acc_c[b] : R.acc_n[b - R.n_acc_c];
         int sub = (charged ? 0 : 2) + (acc_c ? 0 : 1);
@@ -313,14 +315,15 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
         e8[sub] -= (double)hb(dp, hp, ap, sp.hbond_inter[sub]);
       }
     }
-    // receptor donors x ligand acceptors: hpdcc, hpdcn, hpdnc, hpdnn
+    // receptor donors x ligand acceptors: hpdcc, hpdcn, hpdnc, hpdnn (lanes
+    // over the receptor donors)
     int na = li.n_accc + li.n_accn;
-    for (int a = lane; a < na; a += kWave) {
+    for (int a = 0; a < na; ++a) {
       bool acc_c = a < li.n_accc;
       int i = L.sets[(acc_c ? li.accc_off : li.accn_off) + (acc_c ? a : a - li.n_accc)];
       float ap[3];
       lpos(i, ap);
-      for (int b = 0; b < R.n_don_c + R.n_don_n; ++b) {
+      for (int b = lane; b < R.n_don_c + R.n_don_n; b += kWave) {
         bool charged = b < R.n_don_c;
         int2 dh = charged ? R.don_c[b] : R.don_n[b - R.n_don_c];
         int sub = 4 + (charged ? 0 : 2) + (acc_c ? 0 : 1);

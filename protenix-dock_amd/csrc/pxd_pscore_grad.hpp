@@ -447,17 +447,18 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
       }
       return dr * fa;
     };
+    // Lanes run over the receptor acceptors (hundreds), the ligand's few
+    // donors in turn; a donor's and its hydrogen's gradients are summed in
+    // registers, reduced across the wave and added to the shared array once.
     int nd = li.n_donc + li.n_donn;
-    for (int a = lane; a < nd; a += kWave) {
+    for (int a = 0; a < nd; ++a) {
       bool charged = a < li.n_donc;
       int2 dh = L.dons[(charged ? li.donc_off : li.donn_off) + (charged ? a : a - li.n_donc)];
       float dp[3], hp[3], gd[3], gh[3], ga[3];
       lpos(dh.x, dp);
       lpos(dh.y, hp);
-      // the donor's and hydrogen's gradients summed over the receptor
-      // acceptors in registers, added to the shared array once
       float sd[3] = {0.f, 0.f, 0.f}, sh[3] = {0.f, 0.f, 0.f};
-      for (int b = 0; b < R.n_acc_c + R.n_acc_n; ++b) {
+      for (int b = lane; b < R.n_acc_c + R.n_acc_n; b += kWave) {
         bool acc_c = b < R.n_acc_c;
         int j = acc_c ? R.acc_c[b] : R.acc_n[b - R.n_acc_c];
         int sub = (charged ? 0 : 2) + (acc_c ? 0 : 1);
@@ -470,17 +471,23 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
           sh[k] += gh[k] * f;
         }
       }
-      gadd(dh.x, sd, 1.f);
-      gadd(dh.y, sh, 1.f);
+      for (int k = 0; k < 3; ++k) {
+        sd[k] = wave_sum(sd[k]);
+        sh[k] = wave_sum(sh[k]);
+      }
+      if (lane == 0) {
+        gadd(dh.x, sd, 1.f);
+        gadd(dh.y, sh, 1.f);
+      }
     }
     int na = li.n_accc + li.n_accn;
-    for (int a = lane; a < na; a += kWave) {
+    for (int a = 0; a < na; ++a) {  // lanes over the receptor donors
       bool acc_c = a < li.n_accc;
       int i = L.sets[(acc_c ? li.accc_off : li.accn_off) + (acc_c ? a : a - li.n_accc)];
       float ap[3], gd[3], gh[3], ga[3];
       lpos(i, ap);
       float sa[3] = {0.f, 0.f, 0.f};  // the acceptor's gradient over the receptor donors
-      for (int b = 0; b < R.n_don_c + R.n_don_n; ++b) {
+      for (int b = lane; b < R.n_don_c + R.n_don_n; b += kWave) {
         bool charged = b < R.n_don_c;
         int2 dh = charged ? R.don_c[b] : R.don_n[b - R.n_don_c];
         int sub = 4 + (charged ? 0 : 2) + (acc_c ? 0 : 1);
@@ -491,7 +498,8 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         const float f = -w * (float)sp.hbond_sub_coef[sub];
         for (int k = 0; k < 3; ++k) sa[k] += ga[k] * f;
       }
-      gadd(i, sa, 1.f);
+      for (int k = 0; k < 3; ++k) sa[k] = wave_sum(sa[k]);
+      if (lane == 0) gadd(i, sa, 1.f);
     }
     double e = 0.0;
     for (int k = 0; k < 8; ++k) e += sp.hbond_sub_coef[k] * e8[k];
