@@ -183,13 +183,15 @@ __device__ inline double pscore_pose(const DevReceptor& R, const DevLigands& L, 
     double e = 0.0;
     const float sf = (float)sp.hydro_scale_factor, dmin = (float)sp.hydro_dist_min,
                 dmax = (float)sp.hydro_dist_max;
-    for (int a = lane; a < li.n_hydro; a += kWave) {
+    // lanes over the receptor's hydrophobic atoms (about a thousand), the
+    // ligand's in turn
+    for (int a = 0; a < li.n_hydro; ++a) {
       int i = L.sets[li.hydro_off + a];
       float pi[3];
       lpos(i, pi);
       float s1 = s.lj[i].x;
-      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom, then FP64
-      for (int b = 0; b < R.n_hydro; ++b) {
+      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom and lane, then FP64
+      for (int b = lane; b < R.n_hydro; b += kWave) {
         int j = R.hydro[b];
         float pj[3];
         rpos(j, pj);

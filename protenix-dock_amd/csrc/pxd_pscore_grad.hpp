@@ -216,13 +216,15 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
     const float w = w_of(PXD_TERM_HYDROPHOBIC) * (float)sp.hydro_scale;
     const float sf = (float)sp.hydro_scale_factor, dmin = (float)sp.hydro_dist_min,
                 dmax = (float)sp.hydro_dist_max;
-    for (int a = lane; a < li.n_hydro; a += kWave) {
+    // lanes over the receptor's hydrophobic atoms, the ligand's in turn; a
+    // ligand atom's gradient is reduced across the wave and added once
+    for (int a = 0; a < li.n_hydro; ++a) {
       int i = L.sets[li.hydro_off + a];
       float pi[3], gi[3] = {0.f, 0.f, 0.f};
       lpos(i, pi);
       float s1 = s.lj[i].x;
-      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom, then FP64
-      for (int b = 0; b < R.n_hydro; ++b) {
+      float ef = 0.f;  // terms in [0, 1]: FP32 per ligand atom and lane, then FP64
+      for (int b = lane; b < R.n_hydro; b += kWave) {
         int j = R.hydro[b];
         float pj[3];
         rpos(j, pj);
@@ -236,7 +238,8 @@ __device__ inline double pscore_pose_grad(const DevReceptor& R, const DevLigands
         gi[2] += r[2] * f;
       }
       e -= (double)ef;
-      gadd(i, gi, 1.f);
+      for (int k = 0; k < 3; ++k) gi[k] = wave_sum(gi[k]);
+      if (lane == 0) gadd(i, gi, 1.f);
     }
     t[PXD_TERM_HYDROPHOBIC] = e * sp.hydro_scale;
   }
