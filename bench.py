@@ -67,19 +67,24 @@ def workload(n_lig, per_lig, workers):
     return rec, box, ligs, poses
 
 
-def profiled_traffic():
-    """HBM bytes per k_score launch on this workload as measured by the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated) recorded in
-    the committed profile summary; None when the summary is absent."""
+def profiled_value(key):
+    """A figure of the committed k_score profile summary ("key: value" line)
+    as measured by rocprofv3 on this workload; None when absent."""
     path = os.path.join(ROOT, "profiles", "k_score_batched_262144_poses.md")
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return float(line.split(":")[1]), path[len(ROOT) + 1:]
     except OSError:
         pass
     return None, None
+
+
+def profiled_traffic():
+    """HBM bytes per k_score launch from the copy-calibrated FETCH_SIZE /
+    WRITE_SIZE passes recorded in the profile summary."""
+    return profiled_value("traffic_bytes_per_launch")
 
 
 def flops_bytes_per_pose(lig, n_excl):
@@ -448,6 +453,14 @@ def main():
     roof["kernel"] = "k_score"
     roof["algorithmic_bytes_per_launch"] = Bt
     roof["algorithmic_flops_per_launch"] = F
+    # the bound that holds in practice: VALU issue.  SQ_INSTS_VALU per wave
+    # (profile summary) x one wave per pose x 4 cycles per wave64 instruction,
+    # over the SIMDs (4 per CU) at the 2.4 GHz peak engine clock
+    valu, _ = profiled_value("valu_insts_per_wave")
+    if valu:
+        props = torch.cuda.get_device_properties(torch.cuda.current_device())
+        simds = 4 * props.multi_processor_count
+        roof["valu_issue_frac"] = valu * 4 * n_poses / (simds * 2.4e9 * t_s)
 
     out = {
         "metric": "L-BFGS-converged poses/sec + score+grad evals/sec vs plain GPU and CPU",
