@@ -1094,14 +1094,16 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
 __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, double* e,
                                    double* d) {
   const int n = R.tab_n;
-  const double two_pi = 6.283185307179586, h = two_pi / n;
-  double q = fmod((double)theta, two_pi);  // exact reduction
-  if (!(q == q)) {                         // NaN in, NaN out
+  const double two_pi = 6.283185307179586, h = two_pi / n, inv_h = n * (1.0 / two_pi);
+  // reduction to one revolution by floor (a few FP64 instructions; the
+  // rounding of theta / 2 pi moves q by ~1e-15 rad, far below a table step)
+  double q = (double)theta - two_pi * floor((double)theta * (1.0 / two_pi));
+  if (!(q == q)) {  // NaN (or infinite theta) in, NaN out
     *e = *d = q;
     return;
   }
   if (q < 0.0) q += two_pi;
-  double u = q / h;
+  double u = q * inv_h;
   double fl = floor(u);
   double t = u - fl;
   int i0 = (int)fl;
@@ -1114,7 +1116,7 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
   *e = h00 * a.x + h10 * h * a.y + h01 * b.x + h11 * h * b.y;
   double g00 = 6 * t2 - 6 * t, g10 = 3 * t2 - 4 * t + 1, g01 = -6 * t2 + 6 * t,
          g11 = 3 * t2 - 2 * t;
-  *d = (g00 * a.x + g01 * b.x) / h + g10 * a.y + g11 * b.y;
+  *d = (g00 * a.x + g01 * b.x) * inv_h + g10 * a.y + g11 * b.y;
 }
 
 // binding_system_interactions::put_gradients (interaction.cpp:271-298) for
