@@ -291,8 +291,8 @@ typedef struct pxd_ligand_batch pxd_ligand_batch;
  * trees; trees may be NULL) into device CSR arrays.  Every ligand must use a
  * type present in the grid it is scored against.  A tree whose fragment
  * ranges, parents or bond atoms leave the ligand, or whose torsion list
- * names a root fragment or more torsions than non-root fragments, is
- * rejected with PXD_ERR_CONF before any device work. */
+ * names a root fragment or holds n_frags entries or more, is rejected with
+ * PXD_ERR_CONF before any device work. */
 pxd_status pxd_ligand_batch_create(const pxd_mol* ligands, const pxd_frag_tree* trees,
                                    int32_t n, pxd_ligand_batch** out);
 void pxd_ligand_batch_destroy(pxd_ligand_batch* b);
