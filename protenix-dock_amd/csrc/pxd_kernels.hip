@@ -559,7 +559,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         // only waits for the one entry it needs.
         // The arithmetic does not depend on kPre or VPL: every size class
         // gives the same result for the same pose.
-        constexpr int kPre = VPL <= 2 ? 6 : (VPL <= 4 ? 4 : 2);
+        constexpr int kPre = VPL <= 2 ? 8 : (VPL <= 4 ? 6 : 2);
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
         double br[kPre];
 #pragma unroll
