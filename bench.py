@@ -122,7 +122,7 @@ def search_params(seed, steps):
     rotation-vector sigma 0.3 rad, one torsion uniform), local L-BFGS with
     m = 8, at most 300 iterations, |g|inf <= 1e-3, Metropolis T = 1.2."""
     from pxdock_amd import ffi
-    prm = ffi.mc_params(seed=seed, max_nsteps=steps, steps_per_launch=5)
+    prm = ffi.mc_params(seed=seed, max_nsteps=steps, steps_per_launch=10)
     prm.mutation = 1
     prm.trans_sigma = 1.0
     prm.rot_sigma = 0.3
