@@ -289,10 +289,13 @@ typedef struct pxd_ligand_batch pxd_ligand_batch;
 
 /* Packs n ligands (term lists, types, charges, interaction sets and fragment
  * trees; trees may be NULL) into device CSR arrays.  Every ligand must use a
- * type present in the grid it is scored against.  A tree whose fragment
- * ranges, parents or bond atoms leave the ligand, or whose torsion list
- * names a root fragment or holds n_frags entries or more, is rejected with
- * PXD_ERR_CONF before any device work. */
+ * type present in the grid it is scored against.  A tree is rejected with
+ * PXD_ERR_CONF before any device work unless: perm is a permutation of the
+ * atoms; the fragment ranges partition the permuted slots; every parent is a
+ * fragment of the tree and every bond joins an atom of the parent to an atom
+ * of the child; n_torsions equals n_frags minus the number of roots; and
+ * tors_frag lists every non-root fragment exactly once, each after its
+ * parent (free_ligand's pre-order, data.h:78-81). */
 pxd_status pxd_ligand_batch_create(const pxd_mol* ligands, const pxd_frag_tree* trees,
                                    int32_t n, pxd_ligand_batch** out);
 void pxd_ligand_batch_destroy(pxd_ligand_batch* b);

@@ -988,18 +988,55 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
           return fail(PXD_ERR_CONF, "ligand %d: fragment %d bond atom out of range", l, f);
         frag.insert(frag.end(), {s, e, par, T.frag_bond[2 * f], T.frag_bond[2 * f + 1], 0});
       }
-      if (T.n_torsions < 0 || T.n_torsions >= T.n_frags)
-        return fail(PXD_ERR_CONF, "ligand %d: %d torsions for %d fragments", l, T.n_torsions,
-                    T.n_frags);
+      // the fragment ranges partition the permuted slots [0, n), and each
+      // bond joins a parent-side atom of the parent to a mine atom of the child
+      std::vector<int> slot_frag(n, -1);
+      int n_roots = 0;
+      for (int f = 0; f < T.n_frags; ++f) {
+        for (int p = T.frag_start[f]; p < T.frag_end[f]; ++p) {
+          if (slot_frag[p] >= 0)
+            return fail(PXD_ERR_CONF, "ligand %d: fragments %d and %d overlap", l, slot_frag[p], f);
+          slot_frag[p] = f;
+        }
+        if (T.frag_parent[f] < 0) ++n_roots;
+      }
+      for (int p = 0; p < n; ++p)
+        if (slot_frag[p] < 0) return fail(PXD_ERR_CONF, "ligand %d: slot %d in no fragment", l, p);
+      for (int f = 0; f < T.n_frags; ++f) {
+        const int par = T.frag_parent[f];
+        if (par >= 0 && (slot_frag[T.frag_bond[2 * f]] != par || slot_frag[T.frag_bond[2 * f + 1]] != f))
+          return fail(PXD_ERR_CONF, "ligand %d: fragment %d bond does not join it to its parent", l, f);
+      }
+      if (T.n_torsions != T.n_frags - n_roots)
+        return fail(PXD_ERR_CONF, "ligand %d: %d torsions for %d fragments with %d roots", l,
+                    T.n_torsions, T.n_frags, n_roots);
+      // every child fragment once, after its parent (the frames are composed
+      // in this order: a parent's frame must exist before its child's)
+      std::vector<char> placed(T.n_frags, 0);
+      for (int f = 0; f < T.n_frags; ++f) placed[f] = T.frag_parent[f] < 0;
       for (int k = 0; k < T.n_torsions; ++k) {
         const int f = T.tors_frag[k];
         if (f < 0 || f >= T.n_frags || T.frag_parent[f] < 0)
           return fail(PXD_ERR_CONF, "ligand %d: torsion %d moves fragment %d, not a child fragment",
                       l, k, f);
+        if (placed[f])
+          return fail(PXD_ERR_CONF, "ligand %d: fragment %d listed twice in the torsions", l, f);
+        if (!placed[T.frag_parent[f]])
+          return fail(PXD_ERR_CONF, "ligand %d: torsion %d (fragment %d) precedes its parent's", l,
+                      k, f);
+        placed[f] = 1;
         torsf.push_back(f);
       }
       torsf.resize(I.frag_off + T.n_frags, 0);
-      for (int i = 0; i < n; ++i) perm.push_back(T.perm[i]);
+      // perm must be a permutation of the atoms
+      std::vector<char> seen(n, 0);
+      for (int i = 0; i < n; ++i) {
+        const int o = T.perm[i];
+        if (o < 0 || o >= n || seen[o])
+          return fail(PXD_ERR_CONF, "ligand %d: perm is not a permutation of 0..%d", l, n - 1);
+        seen[o] = 1;
+        perm.push_back(o);
+      }
     } else {
       frag.insert(frag.end(), {0, n, -1, 0, 0, 0});
       torsf.push_back(0);

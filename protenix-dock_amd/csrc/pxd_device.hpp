@@ -1090,14 +1090,23 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
   return e;
 }
 
-// Cubic Hermite lookup of leaf k's table at theta (any real theta).
+// Cubic Hermite lookup of leaf k's table at theta (any finite theta; NaN or
+// infinite theta gives NaN).  The table index stays in [0, n) for every input.
+__device__ inline double leaf_reduce(double th) {
+  const double two_pi = 6.283185307179586;
+  // |theta| <= 2^20: reduction by floor (a few FP64 instructions; the
+  // rounding of theta / 2 pi moves q by ~1e-15 rad, far below a table step).
+  // Beyond, the exact remainder: theta / 2 pi then carries no fraction bits
+  // an FMA-contracted floor form could keep inside one revolution.
+  if (fabs(th) <= 1048576.0) return th - two_pi * floor(th * (1.0 / two_pi));
+  return fmod(th, two_pi);
+}
+
 __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, double* e,
                                    double* d) {
   const int n = R.tab_n;
   const double two_pi = 6.283185307179586, h = two_pi / n, inv_h = n * (1.0 / two_pi);
-  // reduction to one revolution by floor (a few FP64 instructions; the
-  // rounding of theta / 2 pi moves q by ~1e-15 rad, far below a table step)
-  double q = (double)theta - two_pi * floor((double)theta * (1.0 / two_pi));
+  double q = leaf_reduce((double)theta);
   if (!(q == q)) {  // NaN (or infinite theta) in, NaN out
     *e = *d = q;
     return;
@@ -1108,6 +1117,7 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
   double t = u - fl;
   int i0 = (int)fl;
   if (i0 >= n) i0 -= n;
+  i0 = i0 < 0 ? 0 : (i0 > n - 1 ? n - 1 : i0);  // bound the read whatever the rounding
   int i1 = i0 + 1 == n ? 0 : i0 + 1;
   double2 a = R.tab[(long long)k * n + i0], b = R.tab[(long long)k * n + i1];
   double t2 = t * t, t3 = t2 * t;

@@ -67,3 +67,72 @@ def test_fragment_tree_is_validated_before_device_work(field, value, msg):
     with pytest.raises(lib.PxdError) as e:
         lib.LigandBatch([lig])
     assert msg in str(e.value).encode()
+
+
+def _tree_ligand():
+    from pxdock_amd import synth
+    lig = synth.make_ligand(3, n_heavy=20, n_torsions=4, n_rings=1)
+    assert lig.tree.n_torsions >= 2
+    return lig
+
+
+def _bad_trees():
+    """(name, mutate(tree), expected message) for trees the ABI must reject."""
+    import numpy as np
+
+    def perm_dup(t):
+        t.perm[1] = t.perm[0]
+
+    def perm_range(t):
+        t.perm[0] = len(t.perm)
+
+    def drop_torsion(t):
+        t.tors_frag = t.tors_frag[:-1].copy()
+
+    def dup_torsion(t):
+        t.tors_frag[-1] = t.tors_frag[0]
+
+    def child_first(t):
+        # a grandchild listed before its parent: pick a fragment whose parent
+        # is not a root and move it to the front
+        order = list(t.tors_frag)
+        for k, f in enumerate(order):
+            if t.frag_parent[f] >= 0 and t.frag_parent[t.frag_parent[f]] >= 0:
+                order.insert(0, order.pop(k))
+                t.tors_frag = np.array(order, dtype=t.tors_frag.dtype)
+                return
+        # no grandchild: make the last child fragment hang off the first child
+        # and list it first
+        a, b = order[0], order[-1]
+        t.frag_parent[b] = a
+        t.frag_bond[b][0] = t.frag_start[a]
+        order.remove(b)
+        t.tors_frag = np.array([b] + order, dtype=t.tors_frag.dtype)
+
+    def overlap(t):
+        t.frag_end[0] = t.frag_end[0] + 1
+
+    def bond_wrong_side(t):
+        f = int(t.tors_frag[0])
+        t.frag_bond[f][1] = t.frag_start[int(t.frag_parent[f])]
+
+    return [("perm_dup", perm_dup, b"not a permutation"),
+            ("perm_range", perm_range, b"not a permutation"),
+            ("drop_torsion", drop_torsion, b"torsions for"),
+            ("dup_torsion", dup_torsion, b"listed twice"),
+            ("child_first", child_first, b"precedes its parent"),
+            ("overlap", overlap, b"overlap"),
+            ("bond_wrong_side", bond_wrong_side, b"does not join")]
+
+
+@pytest.mark.parametrize("name,mutate,msg", _bad_trees(), ids=[b[0] for b in _bad_trees()])
+def test_malformed_fragment_trees_are_rejected(name, mutate, msg):
+    lig = _tree_ligand()
+    try:  # the unmodified tree passes validation (no device here: a HIP error)
+        lib.LigandBatch([lig])
+    except lib.PxdError as e:
+        assert e.status == 3, str(e)
+    mutate(lig.tree)
+    with pytest.raises(lib.PxdError) as e:
+        lib.LigandBatch([lig])
+    assert msg in str(e.value).encode(), str(e.value)

@@ -13,7 +13,14 @@ import pytest
 import pyoracle as O
 from pxdock_amd import ffi, scoring, synth
 
-pytestmark = pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built")
+
+
+@pytest.fixture(autouse=True)
+def _reference_built():
+    """Loads oracle/_ref only when a test of this module runs (never at
+    collection), and skips where it was not built."""
+    if O.ref() is None:
+        pytest.skip("oracle/_ref not built")
 
 
 def test_vdw_table_equals_reference():
