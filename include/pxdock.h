@@ -133,7 +133,9 @@ typedef struct pxd_leaves {
   const int32_t* atoms;     /* movable receptor atoms */
 } pxd_leaves;
 
-/* Grid geometry (grid.h:31-40). */
+/* Grid geometry: the fields of box_config (grid.h:31-40) in its order, with
+ * the point counts as int32 (the reference's index_t is size_t, dtype.h:29-30;
+ * the structs are not layout-compatible: convert field by field). */
 typedef struct pxd_box {
   double init_xyz[3];
   int32_t npoints[3];

@@ -1663,12 +1663,13 @@ static void mc_randomize(const or_system* s, const pxd_frag_tree* t, const pxd_s
 
 /* mutate_inplace (searcher.cpp:189-223), and the Gaussian variant
  * (mutation 1: translation and rotation-vector N(0, sigma^2), one torsion). */
-static void mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, mc_rng* r, int n,
-                      double* x) {
+static int mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, mc_rng* r, int n,
+                     double* x) {
   double cen[3], rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, tors[PXD_MAX_FRAGMENTS] = {0};
+  int which = -1;
   centroid3(x, n, cen);
   if (prm->mutation == 0) {
-    int which = rng_uniform_int(r, 0, t->n_torsions + 1);
+    which = rng_uniform_int(r, 0, t->n_torsions + 1);
     if (which == 0) {
       double dl[3];
       rng_in_sphere(r, dl);
@@ -1681,7 +1682,7 @@ static void mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, mc_rng* 
         gr += sqsum3(v);
       }
       gr = sqrt(gr / n);
-      if (gr < 1e-6) return; /* searcher.cpp:207 */
+      if (gr < 1e-6) return which; /* searcher.cpp:207 */
       double dl[3];
       rng_in_sphere(r, dl);
       scale3(dl, prm->amplitude / gr);
@@ -1703,12 +1704,37 @@ static void mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, mc_rng* 
     if (t->n_torsions > 0) {
       int k = rng_uniform_int(r, 0, t->n_torsions - 1);
       tors[k] = rng_uniform(r, -3.141592653589793, 3.141592653589793);
+      which = k + 2;
     }
   }
   double* y = (double*)malloc(sizeof(double) * 3 * n);
   or_ligand_apply(t, n, x, cen, rot, tors, y);
   memcpy(x, y, sizeof(double) * 3 * n);
   free(y);
+  return which;
+}
+
+int32_t or_mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, int32_t walker,
+                     int32_t ligand, uint32_t* ctr, int32_t n, double* xyz) {
+  mc_rng r = {{(uint32_t)prm->seed, (uint32_t)(prm->seed >> 32) + (uint32_t)walker},
+              (uint32_t)ligand, *ctr};
+  int which = mc_mutate(t, prm, &r, n, xyz);
+  *ctr = r.ctr;
+  return which;
+}
+
+void or_mc_draws(int32_t kind, uint64_t seed, int32_t walker, int32_t ligand, int32_t count,
+                 double* out) {
+  mc_rng r = {{(uint32_t)seed, (uint32_t)(seed >> 32) + (uint32_t)walker}, (uint32_t)ligand, 0u};
+  for (int i = 0; i < count; ++i) {
+    switch (kind) {
+      case 0: out[i] = rng_u01(&r); break;
+      case 1: rng_in_sphere(&r, out + 3 * i); break;
+      case 2: rng_quaternion(&r, out + 4 * i); break;
+      case 3: out[i] = rng_normal(&r); break;
+      default: out[i] = (double)rng_uniform_int(&r, 0, kind - 4); break;
+    }
+  }
 }
 
 /* monte_carlo_searcher::mutate_and_optimize (searcher.cpp:98-187) for one

@@ -164,6 +164,19 @@ int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score
                      int32_t ligand, const double* xyz0, double* best_xyz, double* best_tors,
                      double* best_e, double* best_ps, int32_t* nevals);
 
+/* One mutation of the walker (mutate_inplace, searcher.cpp:189-223, or the
+ * Gaussian variant of pxd_mc_params.mutation 1) applied to xyz in place,
+ * drawing from the walker's stream at counter *ctr (advanced).  Returns the
+ * mutation drawn: 0 translation, 1 rotation, k + 2 a change of torsion k. */
+int32_t or_mc_mutate(const pxd_frag_tree* t, const pxd_mc_params* prm, int32_t walker,
+                     int32_t ligand, uint32_t* ctr, int32_t n, double* xyz);
+/* count draws of the walker stream's distributions (random.cpp:22-49):
+ * kind 0 uniform [0, 1); 1 a point in the unit ball (3 doubles each);
+ * 2 a unit quaternion (4 each); 3 a standard normal; k >= 4 a uniform
+ * integer in [0, k - 4] inclusive. */
+void or_mc_draws(int32_t kind, uint64_t seed, int32_t walker, int32_t ligand, int32_t count,
+                 double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the same source built -O3 -march=x86-64-v4 (AVX-512, FMA contraction): a
+# CPU-baseline variant; the checker is LIB
+NATIVE_LIB = os.path.join(HERE, "liboracle_native.so")
 REF_LIB = os.path.join(HERE, "_ref", "libpxd_ref.so")
 
 from pxdock_amd import ffi  # noqa: E402  (path set up by the caller)
@@ -19,6 +22,15 @@ from pxdock_amd.ffi import f64p, i32p  # noqa: E402
 
 _lib = None
 _ref = None
+_libs = {}
+
+
+def use(path):
+    """Route every later call through the oracle built at path (LIB or
+    NATIVE_LIB).  Only the CPU-baseline timing switches builds."""
+    global _lib, LIB
+    LIB = path
+    _lib = _libs.get(path)
 
 
 def build():
@@ -29,7 +41,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
-            subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+            raise FileNotFoundError(f"{LIB} is not built (make -C oracle)")
         L = C.CDLL(LIB)
         d, i, P = C.c_double, C.c_int32, C.c_void_p
         sig = {
@@ -68,12 +80,15 @@ def lib():
             "or_mc_walker": (i, [P, P, P, P, P, i, i, f64p, f64p, f64p, f64p, f64p, i32p]),
             "or_dof_grad": (None, [P, i, f64p, f64p, f64p, f64p]),
             "or_dof_energy": (d, [P, f64p, f64p, f64p, f64p]),
+            "or_mc_mutate": (i, [P, P, i, i, C.POINTER(C.c_uint32), i, f64p]),
+            "or_mc_draws": (None, [i, C.c_uint64, i, i, i, f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
         _lib = L
+        _libs[LIB] = L
     return _lib
 
 
@@ -99,6 +114,10 @@ def ref():
         L.ref_pscore.restype = d
         L.ref_pscore.argtypes = [P, P, f64p, P, f64p, f64p]
         L.ref_random_draws.argtypes = [C.c_uint, C.c_int, f64p, f64p]
+        L.ref_score_params_compare.restype = C.c_int
+        L.ref_score_params_compare.argtypes = [C.c_char_p, P, C.c_char_p, C.c_size_t]
+        L.ref_pscore_json.restype = d
+        L.ref_pscore_json.argtypes = [C.c_char_p, P, f64p, P, f64p]
         _ref = L
     return _ref
 
@@ -288,3 +307,19 @@ def mc_walker(system, ligand, sp, prm, box, walker, ligand_index, xyz0):
                                dp(bt), dp(be), dp(bp), ip(ne))
     return dict(xyz=bx, tors=bt[:system.n_leaves], energy=float(be[0]), pscore=float(bp[0]),
                 nevals=int(ne[0]), steps=int(steps))
+
+
+def mc_mutate(tree, prm, xyz, walker=1, ligand=0, ctr=0):
+    """One walker mutation of xyz; returns (kind, new xyz, next counter)."""
+    x = f64(xyz).reshape(-1, 3).copy()
+    c = C.c_uint32(ctr)
+    tc = tree.to_c()
+    k = lib().or_mc_mutate(byref(tc), byref(prm), walker, ligand, C.byref(c), len(x), dp(x))
+    return int(k), x, int(c.value)
+
+
+def mc_draws(kind, count, seed=1, walker=1, ligand=0):
+    width = {1: 3, 2: 4}.get(kind, 1)
+    out = np.zeros(count * width)
+    lib().or_mc_draws(kind, seed, walker, ligand, count, dp(out))
+    return out.reshape(count, width) if width > 1 else out

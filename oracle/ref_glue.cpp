@@ -9,6 +9,7 @@
 // data.cpp) includes Boost and is not built.  This glue only converts the
 // pxdock.h descriptors into the reference's own structs and calls the
 // reference code; it restates no algorithm.
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -18,6 +19,7 @@
 #include "bytedock/core/scorer.h"
 #include "bytedock/lib/math.h"
 #include "bytedock/lib/random.h"
+#include "json.hpp"
 #include "pxdock.h"
 
 using namespace bytedock;
@@ -163,9 +165,257 @@ ScoringFunctionParams to_sfp(const pxd_score_params* sp) {
   return f;
 }
 
+// ScoringFunctionParams straight from a pose_selection JSON tree.  Every
+// field is read under the key that spells its member name, which is how the
+// reference's loader names them (factory.cpp:33-211): the stringified member
+// name is the key, so this mapping cannot disagree with the reference's
+// structs.  Sub-objects are read only for active terms (utility.h:24-26).
+#define PXD_RD(node, dst, name) (dst).name = (node)[#name].num()
+#define PXD_RD0(node, dst, name) (dst).name = (node).has(#name) ? (node)[#name].num() : 0.0
+
+bool active_coef(double c) { return c > 1e-5 || c < -1e-5; }
+
+void rd_range2(const pxdock::json& n, IonicInteractionParams& d) {
+  PXD_RD(n, d, dist_min);
+  PXD_RD(n, d, dist_max);
+}
+void rd_range2(const pxdock::json& n, CationPiInteractionParams& d) {
+  PXD_RD(n, d, dist_min);
+  PXD_RD(n, d, dist_max);
+}
+void rd_stack_inter(const pxdock::json& n, PiStackingInteractionParams& d) {
+  PXD_RD(n, d, dist_min);
+  PXD_RD(n, d, dist_max);
+  PXD_RD(n, d, angle_min);
+  PXD_RD(n, d, angle_max);
+  PXD_RD(n, d, sigma_min);
+  PXD_RD(n, d, sigma_max);
+}
+void rd_stack(const pxdock::json& n, PiStackingEnergyParams& d) {
+  PXD_RD(n, d, l5r5_coef);
+  PXD_RD(n, d, l5r6_coef);
+  PXD_RD(n, d, l6r5_coef);
+  PXD_RD(n, d, l6r6_coef);
+  rd_stack_inter(n["l5r5_inter"], d.l5r5_inter);
+  rd_stack_inter(n["l5r6_inter"], d.l5r6_inter);
+  rd_stack_inter(n["l6r5_inter"], d.l6r5_inter);
+  rd_stack_inter(n["l6r6_inter"], d.l6r6_inter);
+}
+void rd_hb_inter(const pxdock::json& n, HbondInteractionParams& d) {
+  PXD_RD(n, d, dist_min);
+  PXD_RD(n, d, dist_max);
+  PXD_RD(n, d, angle_min);
+  PXD_RD(n, d, angle_max);
+  PXD_RD(n, d, dist_center);
+  PXD_RD(n, d, angle_center);
+}
+
+ScoringFunctionParams sfp_from_json(const pxdock::json& n) {
+  ScoringFunctionParams f{};
+  f.ligand_ff = ByteFF1;
+  PXD_RD0(n, f, interaction_coulomb_energy_coef);
+  PXD_RD0(n, f, interaction_vdw_energy_coef);
+  PXD_RD0(n, f, ionic_energy_coef);
+  PXD_RD0(n, f, pi_cationic_energy_coef);
+  PXD_RD0(n, f, f2f_pi_stacking_energy_coef);
+  PXD_RD0(n, f, e2f_pi_stacking_energy_coef);
+  PXD_RD0(n, f, hbond_energy_coef);
+  PXD_RD0(n, f, hydrophobic_energy_coef);
+  PXD_RD0(n, f, torsion_strain_penalty_coef);
+  if (active_coef(f.interaction_vdw_energy_coef)) {
+    const auto& v = n["interaction_vdw_energy"];
+    PXD_RD(v, f.interaction_vdw_energy, scale);
+    PXD_RD(v, f.interaction_vdw_energy, cutoff);
+    const auto& ty = v["types"];
+    f.interaction_vdw_energy.types.resize(ty.size());
+    for (size_t t = 0; t < ty.size(); ++t) {
+      PXD_RD(ty[t], f.interaction_vdw_energy.types[t], sigma_scale);
+      PXD_RD(ty[t], f.interaction_vdw_energy.types[t], epsilon_scale);
+    }
+  }
+  if (active_coef(f.ionic_energy_coef)) {
+    const auto& v = n["ionic_energy"];
+    PXD_RD(v, f.ionic_energy, lc_coef);
+    PXD_RD(v, f.ionic_energy, pc_coef);
+    rd_range2(v["lc_inter"], f.ionic_energy.lc_inter);
+    rd_range2(v["pc_inter"], f.ionic_energy.pc_inter);
+  }
+  if (active_coef(f.pi_cationic_energy_coef)) {
+    const auto& v = n["pi_cationic_energy"];
+    auto& d = f.pi_cationic_energy;
+    PXD_RD(v, d, lc5_coef);
+    PXD_RD(v, d, lc6_coef);
+    PXD_RD(v, d, pc5_coef);
+    PXD_RD(v, d, pc6_coef);
+    rd_range2(v["lc5_inter"], d.lc5_inter);
+    rd_range2(v["lc6_inter"], d.lc6_inter);
+    rd_range2(v["pc5_inter"], d.pc5_inter);
+    rd_range2(v["pc6_inter"], d.pc6_inter);
+  }
+  if (active_coef(f.f2f_pi_stacking_energy_coef))
+    rd_stack(n["f2f_pi_stacking_energy"], f.f2f_pi_stacking_energy);
+  if (active_coef(f.e2f_pi_stacking_energy_coef))
+    rd_stack(n["e2f_pi_stacking_energy"], f.e2f_pi_stacking_energy);
+  if (active_coef(f.hbond_energy_coef)) {
+    const auto& v = n["hbond_energy"];
+    auto& d = f.hbond_energy;
+    PXD_RD(v, d, scale);
+    PXD_RD(v, d, hldcc_coef);
+    PXD_RD(v, d, hldcn_coef);
+    PXD_RD(v, d, hldnc_coef);
+    PXD_RD(v, d, hldnn_coef);
+    PXD_RD(v, d, hpdcc_coef);
+    PXD_RD(v, d, hpdcn_coef);
+    PXD_RD(v, d, hpdnc_coef);
+    PXD_RD(v, d, hpdnn_coef);
+    rd_hb_inter(v["hldcc_inter"], d.hldcc_inter);
+    rd_hb_inter(v["hldcn_inter"], d.hldcn_inter);
+    rd_hb_inter(v["hldnc_inter"], d.hldnc_inter);
+    rd_hb_inter(v["hldnn_inter"], d.hldnn_inter);
+    rd_hb_inter(v["hpdcc_inter"], d.hpdcc_inter);
+    rd_hb_inter(v["hpdcn_inter"], d.hpdcn_inter);
+    rd_hb_inter(v["hpdnc_inter"], d.hpdnc_inter);
+    rd_hb_inter(v["hpdnn_inter"], d.hpdnn_inter);
+  }
+  if (active_coef(f.hydrophobic_energy_coef)) {
+    const auto& v = n["hydrophobic_energy"];
+    PXD_RD(v, f.hydrophobic_energy, scale);
+    PXD_RD(v, f.hydrophobic_energy, dist_min);
+    PXD_RD(v, f.hydrophobic_energy, dist_max);
+    PXD_RD(v, f.hydrophobic_energy, scale_factor);
+  }
+  return f;
+}
+
+// Field-by-field comparison of two parameter sets of the active terms; the
+// name of the first mismatch goes to msg.
+#define PXD_CMP(a, b, path)                                              \
+  do {                                                                   \
+    if (!((a).path == (b).path)) {                                       \
+      if (!nbad) std::snprintf(msg, msg_len, "%s", #path);              \
+      ++nbad;                                                            \
+    }                                                                    \
+  } while (0)
+
+int compare_sfp(const ScoringFunctionParams& a, const ScoringFunctionParams& b, char* msg,
+                size_t msg_len) {
+  int nbad = 0;
+  PXD_CMP(a, b, interaction_coulomb_energy_coef);
+  PXD_CMP(a, b, interaction_vdw_energy_coef);
+  PXD_CMP(a, b, ionic_energy_coef);
+  PXD_CMP(a, b, pi_cationic_energy_coef);
+  PXD_CMP(a, b, f2f_pi_stacking_energy_coef);
+  PXD_CMP(a, b, e2f_pi_stacking_energy_coef);
+  PXD_CMP(a, b, hbond_energy_coef);
+  PXD_CMP(a, b, hydrophobic_energy_coef);
+  PXD_CMP(a, b, torsion_strain_penalty_coef);
+  if (active_coef(a.interaction_vdw_energy_coef)) {
+    PXD_CMP(a, b, interaction_vdw_energy.scale);
+    PXD_CMP(a, b, interaction_vdw_energy.cutoff);
+    for (size_t t = 0; t < kNumVdwTypes; ++t) {
+      PXD_CMP(a, b, interaction_vdw_energy.types[t].sigma_scale);
+      PXD_CMP(a, b, interaction_vdw_energy.types[t].epsilon_scale);
+    }
+  }
+  if (active_coef(a.ionic_energy_coef)) {
+    PXD_CMP(a, b, ionic_energy.lc_coef);
+    PXD_CMP(a, b, ionic_energy.pc_coef);
+    PXD_CMP(a, b, ionic_energy.lc_inter.dist_min);
+    PXD_CMP(a, b, ionic_energy.lc_inter.dist_max);
+    PXD_CMP(a, b, ionic_energy.pc_inter.dist_min);
+    PXD_CMP(a, b, ionic_energy.pc_inter.dist_max);
+  }
+  if (active_coef(a.pi_cationic_energy_coef)) {
+    PXD_CMP(a, b, pi_cationic_energy.lc5_coef);
+    PXD_CMP(a, b, pi_cationic_energy.lc6_coef);
+    PXD_CMP(a, b, pi_cationic_energy.pc5_coef);
+    PXD_CMP(a, b, pi_cationic_energy.pc6_coef);
+    PXD_CMP(a, b, pi_cationic_energy.lc5_inter.dist_min);
+    PXD_CMP(a, b, pi_cationic_energy.lc5_inter.dist_max);
+    PXD_CMP(a, b, pi_cationic_energy.lc6_inter.dist_min);
+    PXD_CMP(a, b, pi_cationic_energy.lc6_inter.dist_max);
+    PXD_CMP(a, b, pi_cationic_energy.pc5_inter.dist_min);
+    PXD_CMP(a, b, pi_cationic_energy.pc5_inter.dist_max);
+    PXD_CMP(a, b, pi_cationic_energy.pc6_inter.dist_min);
+    PXD_CMP(a, b, pi_cationic_energy.pc6_inter.dist_max);
+  }
+#define PXD_CMP_STACK(E, K)                  \
+  PXD_CMP(a, b, E.K##_coef);                 \
+  PXD_CMP(a, b, E.K##_inter.dist_min);       \
+  PXD_CMP(a, b, E.K##_inter.dist_max);       \
+  PXD_CMP(a, b, E.K##_inter.angle_min);      \
+  PXD_CMP(a, b, E.K##_inter.angle_max);      \
+  PXD_CMP(a, b, E.K##_inter.sigma_min);      \
+  PXD_CMP(a, b, E.K##_inter.sigma_max)
+  if (active_coef(a.f2f_pi_stacking_energy_coef)) {
+    PXD_CMP_STACK(f2f_pi_stacking_energy, l5r5);
+    PXD_CMP_STACK(f2f_pi_stacking_energy, l5r6);
+    PXD_CMP_STACK(f2f_pi_stacking_energy, l6r5);
+    PXD_CMP_STACK(f2f_pi_stacking_energy, l6r6);
+  }
+  if (active_coef(a.e2f_pi_stacking_energy_coef)) {
+    PXD_CMP_STACK(e2f_pi_stacking_energy, l5r5);
+    PXD_CMP_STACK(e2f_pi_stacking_energy, l5r6);
+    PXD_CMP_STACK(e2f_pi_stacking_energy, l6r5);
+    PXD_CMP_STACK(e2f_pi_stacking_energy, l6r6);
+  }
+#define PXD_CMP_HB(K)                              \
+  PXD_CMP(a, b, hbond_energy.K##_coef);            \
+  PXD_CMP(a, b, hbond_energy.K##_inter.dist_min);  \
+  PXD_CMP(a, b, hbond_energy.K##_inter.dist_max);  \
+  PXD_CMP(a, b, hbond_energy.K##_inter.angle_min); \
+  PXD_CMP(a, b, hbond_energy.K##_inter.angle_max); \
+  PXD_CMP(a, b, hbond_energy.K##_inter.dist_center); \
+  PXD_CMP(a, b, hbond_energy.K##_inter.angle_center)
+  if (active_coef(a.hbond_energy_coef)) {
+    PXD_CMP(a, b, hbond_energy.scale);
+    PXD_CMP_HB(hldcc);
+    PXD_CMP_HB(hldcn);
+    PXD_CMP_HB(hldnc);
+    PXD_CMP_HB(hldnn);
+    PXD_CMP_HB(hpdcc);
+    PXD_CMP_HB(hpdcn);
+    PXD_CMP_HB(hpdnc);
+    PXD_CMP_HB(hpdnn);
+  }
+  if (active_coef(a.hydrophobic_energy_coef)) {
+    PXD_CMP(a, b, hydrophobic_energy.scale);
+    PXD_CMP(a, b, hydrophobic_energy.dist_min);
+    PXD_CMP(a, b, hydrophobic_energy.dist_max);
+    PXD_CMP(a, b, hydrophobic_energy.scale_factor);
+  }
+  return nbad;
+}
+
 }  // namespace
 
 extern "C" {
+
+// The parameters the reference would read from a pose_selection JSON tree
+// (json_text) against the pxd_score_params the host loader produced from the
+// same text: the number of fields that differ (-1 on a JSON error); msg gets
+// the first differing field or the error.
+int ref_score_params_compare(const char* json_text, const pxd_score_params* sp, char* msg,
+                             size_t msg_len) {
+  try {
+    ScoringFunctionParams a = sfp_from_json(pxdock::json::parse(json_text));
+    return compare_sfp(a, to_sfp(sp), msg, msg_len);
+  } catch (const std::exception& e) {
+    std::snprintf(msg, msg_len, "%s", e.what());
+    return -1;
+  }
+}
+
+// ref_pscore with the parameters read by the reference's member names from
+// a pose_selection JSON tree (see sfp_from_json).
+double ref_pscore_json(const char* json_text, const pxd_mol* rec, const double* rec_xyz,
+                       const pxd_mol* lig, const double* lig_xyz) {
+  force_field_params rf = to_ffp(rec), lf = to_ffp(lig);
+  molecule_pose rx = to_pose(rec_xyz, rec->n_atoms), lx = to_pose(lig_xyz, lig->n_atoms);
+  root_scorer root("pscore", sfp_from_json(pxdock::json::parse(json_text)));
+  instant_cache memo = root.precalculate(rx, rf, lx, lf);
+  return root.combine(root.summary(rx, rf, lx, lf, &memo));
+}
 
 // kVdwTypeDatabase (constant.h:29-130)
 void ref_vdw_table(double* sigma, double* epsilon) {

@@ -1174,6 +1174,64 @@ static pxd_status check_grid_types(const pxd_grid* g, const pxd_ligand_batch* b)
   return PXD_OK;
 }
 
+}  // extern "C"
+
+namespace pxd {
+
+// Argument checks of pxd_lbfgs_batch and its launch arguments (all but the
+// queue heads and the history storage), shared with the entry point that
+// allocates nothing.  *n_slots: the work slots the tuning asks for (at most
+// one per pose).  n_poses == 0 returns PXD_OK with nothing to launch.
+pxd_status lbfgs_prepare(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                         const pxd_lbfgs_params* prm, int32_t n_poses, const int32_t* pose_lig,
+                         const int32_t* pose_atom_off, float* xyz, float* tors, double* energy,
+                         int32_t* nevals, int32_t* converged, LbfgsArgs* a, int* n_slots) {
+  if (!r || !b || !prm) return fail(PXD_ERR_ARG, "lbfgs: null argument");
+  if (n_poses < 0) return fail(PXD_ERR_ARG, "lbfgs: n_poses < 0");
+  if (n_poses == 0) return PXD_OK;
+  if (!pose_lig || !pose_atom_off || !xyz || !energy || !nevals || !converged)
+    return fail(PXD_ERR_ARG, "lbfgs: null pose array");
+  if (r->n_leaves > 0 && !tors) return fail(PXD_ERR_ARG, "lbfgs: tors required (receptor has leaves)");
+  if (prm->history < 1 || prm->history > kMaxHistory)
+    return fail(PXD_ERR_RANGE, "lbfgs: history %d outside 1..%d", prm->history, kMaxHistory);
+  if (prm->max_iter < 0 || prm->max_eval < 0 || prm->max_ls < 0)
+    return fail(PXD_ERR_ARG, "lbfgs: negative iteration limits");
+  if (prm->dof_mode && !b->has_trees)
+    return fail(PXD_ERR_STATE, "lbfgs: rigid-body + torsion variables need fragment trees");
+  pxd_status st = check_grid_types(g, b);
+  if (st != PXD_OK) return st;
+  int dev = 0, ncu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  int spc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    spc = g_tune.lbfgs_slots_per_cu;
+  }
+  *n_slots = std::min(n_poses, std::max(1, ncu) * spc);
+  a->prm = {prm->history, prm->max_iter,  prm->max_eval,      prm->max_ls,
+            prm->lr,      prm->tol_grad,  prm->tol_change,    prm->c1,
+            prm->c2,      prm->ls_tol_change, prm->gtd_threshold, prm->dof_mode ? 1 : 0};
+  a->n_poses = n_poses;
+  a->pose_lig = pose_lig;
+  a->pose_off = pose_atom_off;
+  a->xyz = xyz;
+  a->tors = tors;
+  a->energy = energy;
+  a->nevals = nevals;
+  a->converged = converged;
+  a->counter = nullptr;
+  a->hist = nullptr;
+  a->max_atoms = b->max_atoms;
+  return PXD_OK;
+}
+
+const DevGrid* grid_device(const pxd_grid* g) { return g ? grid_dev_ptr(g) : nullptr; }
+
+}  // namespace pxd
+
+extern "C" {
+
 pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* g,
                                 const pxd_ligand_batch* b, int32_t n_poses,
                                 const int32_t* pose_lig, const int32_t* pose_atom_off,
@@ -1199,30 +1257,12 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
                            const pxd_lbfgs_params* prm, int32_t n_poses, const int32_t* pose_lig,
                            const int32_t* pose_atom_off, float* xyz, float* tors,
                            double* energy, int32_t* nevals, int32_t* converged, void* stream) {
-  if (!r || !b || !prm) return fail(PXD_ERR_ARG, "lbfgs: null argument");
-  if (n_poses < 0) return fail(PXD_ERR_ARG, "lbfgs: n_poses < 0");
-  if (n_poses == 0) return PXD_OK;
-  if (!pose_lig || !pose_atom_off || !xyz || !energy || !nevals || !converged)
-    return fail(PXD_ERR_ARG, "lbfgs: null pose array");
-  if (r->n_leaves > 0 && !tors) return fail(PXD_ERR_ARG, "lbfgs: tors required (receptor has leaves)");
-  if (prm->history < 1 || prm->history > kMaxHistory)
-    return fail(PXD_ERR_RANGE, "lbfgs: history %d outside 1..%d", prm->history, kMaxHistory);
-  if (prm->max_iter < 0 || prm->max_eval < 0 || prm->max_ls < 0)
-    return fail(PXD_ERR_ARG, "lbfgs: negative iteration limits");
-  if (prm->dof_mode && !b->has_trees)
-    return fail(PXD_ERR_STATE, "lbfgs: rigid-body + torsion variables need fragment trees");
-  pxd_status st = check_grid_types(g, b);
-  if (st != PXD_OK) return st;
+  LbfgsArgs a;
+  int n_slots = 0;
+  pxd_status st = pxd::lbfgs_prepare(r, g, b, prm, n_poses, pose_lig, pose_atom_off, xyz, tors,
+                                     energy, nevals, converged, &a, &n_slots);
+  if (st != PXD_OK || n_poses == 0) return st;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int dev = 0, ncu = 0;
-  HIP_TRY(hipGetDevice(&dev));
-  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  int spc;
-  {
-    std::lock_guard<std::mutex> lk(g_tune.mu);
-    spc = g_tune.lbfgs_slots_per_cu;
-  }
-  int n_slots = std::min(n_poses, std::max(1, ncu) * spc);
   size_t per_slot = lbfgs_slot_floats(prm->history);
   std::lock_guard<std::mutex> lock(g_ws.mu);
   if (g_ws.hist_floats < per_slot * n_slots) {
@@ -1234,21 +1274,8 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   }
   if (!g_ws.counter) HIP_TRY(hipMalloc(&g_ws.counter, sizeof(int) * 64));
   HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int) * kLbfgsClasses, s));
-  LbfgsArgs a;
-  a.prm = {prm->history, prm->max_iter,  prm->max_eval,      prm->max_ls,
-           prm->lr,      prm->tol_grad,  prm->tol_change,    prm->c1,
-           prm->c2,      prm->ls_tol_change, prm->gtd_threshold, prm->dof_mode ? 1 : 0};
-  a.n_poses = n_poses;
-  a.pose_lig = pose_lig;
-  a.pose_off = pose_atom_off;
-  a.xyz = xyz;
-  a.tors = tors;
-  a.energy = energy;
-  a.nevals = nevals;
-  a.converged = converged;
   a.counter = g_ws.counter;
   a.hist = g_ws.hist;
-  a.max_atoms = b->max_atoms;
   launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
   HIP_TRY(hipGetLastError());
   // the shared workspace may be reused by the next call: finish this one

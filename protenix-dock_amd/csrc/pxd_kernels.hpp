@@ -177,3 +177,12 @@ void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, cons
                      const float* xyz, const float* grad, float* out, hipStream_t st);
 
 }  // namespace pxd
+
+namespace pxd {
+// Shared by pxd_lbfgs_batch and pxd_lbfgs_batch_noalloc (pxd_api.cpp).
+pxd_status lbfgs_prepare(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                         const pxd_lbfgs_params* prm, int32_t n_poses, const int32_t* pose_lig,
+                         const int32_t* pose_atom_off, float* xyz, float* tors, double* energy,
+                         int32_t* nevals, int32_t* converged, LbfgsArgs* a, int* n_slots);
+const DevGrid* grid_device(const pxd_grid* g);
+}  // namespace pxd

@@ -283,37 +283,7 @@ def test_gpu_grid_end_to_end_small_box():
 
 
 # ---------------------------------------------------------------- L-BFGS
-
-def test_gpu_lbfgs_matches_oracle(world):
-    """Persistent batched L-BFGS (reference parameters, relax = 40 iterations) vs the
-    FP64 oracle from the same starts: converged final energies agree to 1e-3 kcal/mol
-    (or 1e-4 relative) for most starts; FP32-vs-FP64 line-search branch flips account
-    for the rest, and every GPU result must still be a descent from its start."""
-    og = _oracle_grid(world)
-    pose_lig, xyz, tors = _poses(world, 6, 21, spread=2.0)
-    B = lib.LigandBatch(world["ligs"])
-    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
-    e0, _, _ = lib.score_grad(world["R"], world["G"], P)
-    prm = ffi.lbfgs_params(40)
-    e, ne, cv = lib.lbfgs(world["R"], world["G"], P, prm)
-    e, ne, e0 = e.cpu().numpy(), ne.cpu().numpy(), e0.cpu().numpy()
-    systems = [O.System(world["rec"], lig, og) for lig in world["ligs"]]
-    agree = 0
-    rows = []
-    for p in range(P.n_poses):
-        out = systems[pose_lig[p]].lbfgs(xyz[p], tors[p], prm)
-        if abs(e[p] - out["energy"]) <= max(1e-3, 1e-4 * abs(out["energy"])):
-            agree += 1
-        rows.append((p, e0[p], e[p], out["energy"], int(ne[p]), out["nevals"]))
-        assert np.isfinite(e[p]) and e[p] <= e0[p] + 1e-6 * abs(e0[p])
-        # max_eval is tested at iteration start only (optimizer.cpp:454): a line
-        # search may add up to max_ls evaluations, plus the final one
-        assert 1 <= ne[p] <= prm.max_eval + prm.max_ls + 1
-    print("\nL-BFGS parity (pose, E0, E_gpu, E_oracle, nevals_gpu, nevals_oracle):")
-    for r in rows:
-        print("  ", r)
-    print(f"agree {agree}/{P.n_poses}")
-    assert agree >= 0.5 * P.n_poses, (agree, P.n_poses)
+# Same-start parity of the batched L-BFGS is tested in test_gpu_lbfgs_parity.py.
 
 
 # ---------------------------------------------------------------- pscore

@@ -7,6 +7,8 @@ pscore leaves, pair terms and geometry on realistic synthetic inputs that no
 gtest vector covers.  Skipped where the reference tree was not available to
 build _ref.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -154,3 +156,71 @@ def test_random_samplers_match_reference_properties():
     ours = synth.random_quaternion(np.random.default_rng(1), n)
     # both are uniform on SO(3): the first component's |a| has mean 2/pi * ... compare moments
     assert abs(np.abs(ours[:, 0]).mean() - np.abs(qs[:, 0]).mean()) < 0.03
+
+
+def _perturbed_config(seed, rel=0.03):
+    """The default scoring JSON with every number of pose_selection scaled by
+    its own factor exp(N(0, rel^2)): no two fields keep equal values, so a
+    loader that read one field into another's place is seen."""
+    import json
+    with open(scoring.DEFAULT_CONFIG) as fh:
+        cfg = json.load(fh)
+    rng = np.random.default_rng(seed)
+
+    def walk(node):
+        if isinstance(node, dict):
+            return {k: (v if k == "ligand_ff" else walk(v)) for k, v in node.items()}
+        if isinstance(node, list):
+            return [walk(v) for v in node]
+        if isinstance(node, (int, float)) and not isinstance(node, bool):
+            return float(node) * float(np.exp(rng.normal(0.0, rel)))
+        return node
+    return walk(cfg["pose_selection"])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_scoring_loader_reads_every_field_the_reference_reads(seed):
+    """scoring.params_from_dict against the reference's own field names
+    (factory.cpp:33-211): the glue fills ScoringFunctionParams by reading each
+    member under the key that spells its name, the host loader's
+    pxd_score_params is mapped into the same struct, and every field of every
+    active term must be equal."""
+    import json
+    node = _perturbed_config(seed)
+    sp = scoring.params_from_dict(node)
+    msg = C.create_string_buffer(256)
+    n_bad = O.ref().ref_score_params_compare(json.dumps(node).encode(), O.byref(sp), msg, 256)
+    assert n_bad == 0, (n_bad, msg.value)
+
+
+def test_scoring_loader_field_slip_is_detected():
+    """The comparison sees a single swapped pair (hbond dist_center and
+    angle_center of one class) and a single shifted index (the l5r6 and l6r5
+    stacking coefficients)."""
+    import json
+    node = _perturbed_config(5)
+    msg = C.create_string_buffer(256)
+    sp = scoring.params_from_dict(node)
+    sp.hbond_inter[3][4], sp.hbond_inter[3][5] = sp.hbond_inter[3][5], sp.hbond_inter[3][4]
+    assert O.ref().ref_score_params_compare(json.dumps(node).encode(), O.byref(sp), msg, 256) == 2
+    assert b"hldnn_inter.dist_center" in msg.value
+    sp = scoring.params_from_dict(node)
+    sp.f2f_sub_coef[1], sp.f2f_sub_coef[2] = sp.f2f_sub_coef[2], sp.f2f_sub_coef[1]
+    assert O.ref().ref_score_params_compare(json.dumps(node).encode(), O.byref(sp), msg, 256) == 2
+
+
+def test_pscore_from_json_equals_reference(receptor):
+    """End to end: the oracle scoring with the host loader's parameters equals
+    the reference scorer with parameters it read by name, on a perturbed
+    configuration."""
+    import json
+    node = _perturbed_config(9)
+    sp = scoring.params_from_dict(node)
+    text = json.dumps(node).encode()
+    lig = synth.make_ligand(1234, n_heavy=34, n_torsions=6, n_rings=3)
+    rc, lc = receptor.to_c(), lig.to_c()
+    for x in _near_pocket_poses(receptor, lig, 8, 4):
+        s1, _ = O.pscore(sp, receptor, receptor.xyz, lig, x)
+        s2 = O.ref().ref_pscore_json(text, O.byref(rc), O.dp(O.f64(receptor.xyz)), O.byref(lc),
+                                     O.dp(O.f64(x)))
+        assert s1 == pytest.approx(s2, rel=1e-10, abs=1e-10)
