@@ -332,6 +332,26 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* grid,
                            double* energy, int32_t* nevals, int32_t* converged,
                            void* stream);
 
+/* pxd_lbfgs_batch for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and checks as
+ * pxd_lbfgs_batch, but nothing is allocated or freed and the call does not
+ * wait for the stream: the work-queue heads and L-BFGS histories live in a
+ * fixed scratch array of PXD_NOALLOC_SCRATCH_MB MiB per device, held until the
+ * process ends.  The work slots are as many as the tuning asks for and the
+ * scratch holds (history 8: 2,335; history 100: 186), so long
+ * histories run fewer poses at a time; results are identical either way.
+ * Every call on a device shares that device's scratch: calls whose launches
+ * overlap on one device (two streams, or two host threads) race, so order
+ * them on one stream. */
+#define PXD_NOALLOC_SCRATCH_MB 64
+pxd_status pxd_lbfgs_batch_noalloc(const pxd_receptor* r, const pxd_grid* grid,
+                                   const pxd_ligand_batch* b, const pxd_lbfgs_params* prm,
+                                   int32_t n_poses, const int32_t* pose_lig,
+                                   const int32_t* pose_atom_off, float* xyz, float* tors,
+                                   double* energy, int32_t* nevals, int32_t* converged,
+                                   void* stream);
+
 /* The plain GPU form of pxd_lbfgs_batch, kept as the comparison baseline:
  * the optimiser runs on the host (FP64) and every energy evaluation is one
  * launch of the same score+grad kernel for one pose, followed by a copy back

@@ -63,6 +63,7 @@ def load():
         "pxd_lbfgs_batch": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_pscore_batch": (i, [P, P, P, i, P, P, P, P, P, P, P]),
         "pxd_lbfgs_plain": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
+        "pxd_lbfgs_batch_noalloc": (i, [P, P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_receptor_leaf_tables": (i, [P, i]),
         "pxd_receptor_info": (i, [P, i32p, i32p, f64p, f64p]),
         "pxd_mc_default_params": (None, [P]),
@@ -266,6 +267,25 @@ def lbfgs(receptor, grid, poses, params):
                                  _ref(params), poses.n_poses, _dev(poses.d_lig), _dev(poses.d_off),
                                  _dev(poses.xyz), _dev(poses.tors), _dev(e), _dev(ne), _dev(cv),
                                  _stream()))
+    return e, ne, cv
+
+
+def lbfgs_noalloc(receptor, grid, poses, params, out=None):
+    """pxd_lbfgs_batch_noalloc: as lbfgs(), with the library allocating
+    nothing; out = (energy, nevals, converged) device tensors to fill (made
+    here when None).  Returns without waiting for the stream."""
+    import torch
+    dev = poses.xyz.device
+    if out is None:
+        out = (torch.empty(poses.n_poses, dtype=torch.float64, device=dev),
+               torch.empty(poses.n_poses, dtype=torch.int32, device=dev),
+               torch.empty(poses.n_poses, dtype=torch.int32, device=dev))
+    e, ne, cv = out
+    check(load().pxd_lbfgs_batch_noalloc(receptor.h, grid.h if grid is not None else None,
+                                         poses.batch.h, _ref(params), poses.n_poses,
+                                         _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
+                                         _dev(poses.tors), _dev(e), _dev(ne), _dev(cv),
+                                         _stream()))
     return e, ne, cv
 
 
