@@ -314,6 +314,8 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   std::vector<int> fixed;
   std::vector<float4> fxyzq;
   std::vector<float2> flj;
+  std::vector<double4> fxyzq64;
+  std::vector<double2> flj64;
   for (int i = 0; i < n; ++i) {
     xyzq[i] = make_float4((float)xyz[3 * i], (float)xyz[3 * i + 1], (float)xyz[3 * i + 2],
                           (float)m->charge[i]);
@@ -322,6 +324,8 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
       fixed.push_back(i);
       fxyzq.push_back(xyzq[i]);
       flj.push_back(lj[i]);
+      fxyzq64.push_back(make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], m->charge[i]));
+      flj64.push_back(make_double2(m->sigma[i], std::sqrt(m->epsilon[i])));
     }
   }
   // constant part: terms with no movable atom (FP64 on the host)
@@ -514,7 +518,8 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
 
   Packer P;
   size_t o_xyzq = P.add(xyzq), o_lj = P.add(lj), o_fixed = P.add(fixed), o_fxyzq = P.add(fxyzq),
-         o_flj = P.add(flj), o_lorg = P.add(lorg), o_lax = P.add(lax),
+         o_flj = P.add(flj), o_fxyzq64 = P.add(fxyzq64), o_flj64 = P.add(flj64),
+         o_lorg = P.add(lorg), o_lax = P.add(lax),
          o_movleaf = P.add(mov_leaf), o_movatom = P.add(mov_atom), o_bab = P.add(bab),
          o_bkr = P.add(bkr), o_aabc = P.add(aabc), o_akt = P.add(akt), o_dabcd = P.add(dabcd),
          o_dper = P.add(dper), o_dkp = P.add(dkp), o_rpseg = P.add(rp_seg),
@@ -587,6 +592,8 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   D.e_const = e_const;
   D.fixed_xyzq = dptr<float4>(B, o_fxyzq);
   D.fixed_lj = dptr<float2>(B, o_flj);
+  D.fixed_xyzq64 = dptr<double4>(B, o_fxyzq64);
+  D.fixed_lj64 = dptr<double2>(B, o_flj64);
   D.bond_seg = dptr<int>(B, o_bseg);
   D.angle_seg = dptr<int>(B, o_aseg);
   D.dih_seg = dptr<int>(B, o_dseg);
@@ -700,12 +707,22 @@ pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32
   pxd_grid* g = nullptr;
   pxd_status st = grid_alloc(r, box, types, n_types, slope, &g);
   if (st != PXD_OK) return st;
-  std::vector<float2> tlj(n_types > 0 ? n_types : 1);
-  for (int t = 0; t < n_types; ++t)
-    tlj[t] = make_float2((float)kGaffSigma[types[t]], (float)(4.0 * std::sqrt(kGaffEpsilon[types[t]])));
-  float2* d_tlj = nullptr;
-  hipError_t he = hipMalloc(&d_tlj, tlj.size() * sizeof(float2));
-  if (he == hipSuccess) he = hipMemcpy(d_tlj, tlj.data(), tlj.size() * sizeof(float2), hipMemcpyHostToDevice);
+  const int nt1 = n_types > 0 ? n_types : 1;
+  std::vector<float2> tlj(nt1);
+  std::vector<double2> tlj64(nt1);
+  for (int t = 0; t < n_types; ++t) {
+    tlj64[t] = make_double2(kGaffSigma[types[t]], 4.0 * std::sqrt(kGaffEpsilon[types[t]]));
+    tlj[t] = make_float2((float)tlj64[t].x, (float)tlj64[t].y);
+  }
+  // one staging allocation: FP64 type parameters (256-byte aligned), then FP32
+  char* d_tlj_mem = nullptr;
+  const size_t o32 = (sizeof(double2) * nt1 + 255) & ~size_t(255);
+  hipError_t he = hipMalloc(&d_tlj_mem, o32 + sizeof(float2) * nt1);
+  double2* d_tlj64 = reinterpret_cast<double2*>(d_tlj_mem);
+  float2* d_tlj = reinterpret_cast<float2*>(d_tlj_mem + o32);
+  if (he == hipSuccess)
+    he = hipMemcpy(d_tlj64, tlj64.data(), sizeof(double2) * nt1, hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMemcpy(d_tlj, tlj.data(), sizeof(float2) * nt1, hipMemcpyHostToDevice);
   if (he != hipSuccess) {
     pxd_grid_destroy(g);
     return fail(PXD_ERR_HIP, "grid: %s", hipGetErrorString(he));
@@ -713,14 +730,19 @@ pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32
   GridBuildArgs a;
   a.atoms = r->dev.fixed_xyzq;
   a.alj = r->dev.fixed_lj;
+  a.atoms64 = r->dev.fixed_xyzq64;
+  a.alj64 = r->dev.fixed_lj64;
   a.n_atoms = r->dev.n_fixed;
   a.tlj = d_tlj;
+  a.tlj64 = d_tlj64;
   a.n_types = n_types;
   for (int d = 0; d < 3; ++d) {
     a.init[d] = g->dev.init[d];
+    a.init64[d] = g->box.init_xyz[d];
     a.np[d] = g->dev.np[d];
   }
   a.spacing = g->dev.spacing;
+  a.spacing64 = g->box.spacing;
   a.P = g->dev.P;
   a.maps = const_cast<float4*>(g->dev.maps);
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -733,7 +755,7 @@ pxd_status pxd_grid_build(const pxd_receptor* r, const pxd_box* box, const int32
   launch_grid_build(a, tpp, s);
   he = hipGetLastError();
   if (he == hipSuccess) he = hipStreamSynchronize(s);
-  (void)hipFree(d_tlj);
+  (void)hipFree(d_tlj_mem);
   if (he != hipSuccess) {
     pxd_grid_destroy(g);
     return fail(PXD_ERR_HIP, "grid build: %s", hipGetErrorString(he));

@@ -72,6 +72,8 @@ struct DevReceptor {
   // fixed atoms compacted for the grid build
   const float4* fixed_xyzq;
   const float2* fixed_lj;
+  const double4* fixed_xyzq64;  // the same in FP64 (x, y, z, q) and (sigma, sqrt eps)
+  const double2* fixed_lj64;
   // FP64 energy of receptor terms touching no movable atom (pose independent)
   double e_const;
   // Term lists above are ordered by leaf class: the terms whose movable atoms

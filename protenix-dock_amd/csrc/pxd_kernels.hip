@@ -25,53 +25,120 @@ __device__ inline int xcd_block(int b, int nb) {
 // point, Coulomb for a unit probe and LJ per ligand type, each with the probe
 // gradient, summed over the fixed (non-movable) receptor atoms.  One thread
 // per point, TG types per pass, receptor atoms staged through LDS tiles.
-// Distances and reciprocals use correctly rounded sqrt/division: a map value
-// or gradient is a sum of ~N_rec terms that cancel heavily near atoms, and
-// the build runs once per receptor.
+// Distances and reciprocals use correctly rounded sqrt/division.
+//
+// Near atoms in FP64.  A map value or gradient is a sum of ~N_rec terms, and
+// next to an atom single terms reach 1e5-1e6 while the channel they land in
+// may sum to a few units: an FP32 term (and the FP32 coordinates and
+// parameters it is computed from) then carries more error than the whole
+// sum may have.  So an atom closer to a point than R = (sigma_atom +
+// largest type sigma) / 2 -- every term of size above ~eps of the pair, and
+// the steep Coulomb core -- is left out of the FP32 sweep for that point and
+// summed afterwards in FP64 from FP64 coordinates, charges and LJ
+// parameters.  The candidates are the atoms within R of the block's bounding
+// box, collected per block in LDS while the tiles stream past (a few dozen;
+// beyond kNearCap the rest stay FP32).  Sweep and FP64 pass test "closer
+// than R" with the same uncontracted FP32 expression, so every term is
+// counted exactly once.
+
+constexpr int kGridTile = 256;
+constexpr int kNearCap = 1024;
+
+__device__ inline float dist2_rn(float vx, float vy, float vz) {
+  return __fadd_rn(__fadd_rn(__fmul_rn(vx, vx), __fmul_rn(vy, vy)), __fmul_rn(vz, vz));
+}
 
 template <int TG>
-__global__ __launch_bounds__(256) void k_grid(const float4* __restrict__ atoms,
-                                             const float2* __restrict__ alj, int n_atoms,
-                                             const float2* __restrict__ tlj, int n_types,
-                                             int type0, float ix, float iy, float iz, int ny,
-                                             int nz, float spacing, long long P,
-                                             float4* __restrict__ maps, int with_coulomb) {
-  constexpr int kTile = 256;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_grid(GridBuildArgs A, int type0,
+                                                                                 int with_coulomb) {
+  constexpr int kTile = kGridTile;
   __shared__ float4 s_at[kTile];
   __shared__ float2 s_lj[kTile];
+  __shared__ float s_r2[kTile];      // R^2 of a near candidate, -1 otherwise
   __shared__ float2 s_tlj[TG];
+  __shared__ float4 s_nat[kNearCap]; // candidate x, y, z, R^2
+  __shared__ int s_nidx[kNearCap];
+  __shared__ int s_nn;
+  const int ny = A.np[1], nz = A.np[2];
+  const float ix = A.init[0], iy = A.init[1], iz = A.init[2], spacing = A.spacing;
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  int nt = n_types - type0 < TG ? n_types - type0 : TG;
-  if (threadIdx.x < TG) s_tlj[threadIdx.x] = threadIdx.x < nt ? tlj[type0 + threadIdx.x] : make_float2(0.f, 0.f);
+  int nt = A.n_types - type0 < TG ? A.n_types - type0 : TG;
+  if (threadIdx.x < TG)
+    s_tlj[threadIdx.x] = threadIdx.x < nt ? A.tlj[type0 + threadIdx.x] : make_float2(0.f, 0.f);
+  if (threadIdx.x == 0) s_nn = 0;
   int k = (int)(p % nz);
   int j = (int)((p / nz) % ny);
   int i = (int)(p / ((long long)nz * ny));
   // grid.cpp:165-169 (init + spacing * index)
-  float px = ix + spacing * i, py = iy + spacing * j, pz = iz + spacing * k;
+  const float px = ix + spacing * i, py = iy + spacing * j, pz = iz + spacing * k;
+  // the block's bounding box (its points are consecutive in x-major order)
+  // and the largest type sigma, in LDS: only the tile loads read them
+  __shared__ float s_bb[7];
+  if (threadIdx.x == 0) {
+    long long p0 = (long long)blockIdx.x * blockDim.x;
+    long long p1 = p0 + blockDim.x - 1 < A.P - 1 ? p0 + blockDim.x - 1 : A.P - 1;
+    long long nyz = (long long)ny * nz;
+    int i0 = (int)(p0 / nyz), i1 = (int)(p1 / nyz), j0 = 0, j1 = ny - 1, k0 = 0, k1 = nz - 1;
+    if (i0 == i1) {
+      j0 = (int)((p0 / nz) % ny);
+      j1 = (int)((p1 / nz) % ny);
+      if (j0 == j1) {
+        k0 = (int)(p0 % nz);
+        k1 = (int)(p1 % nz);
+      }
+    }
+    s_bb[0] = ix + spacing * i0; s_bb[3] = ix + spacing * i1;
+    s_bb[1] = iy + spacing * j0; s_bb[4] = iy + spacing * j1;
+    s_bb[2] = iz + spacing * k0; s_bb[5] = iz + spacing * k1;
+    // over all requested types, so every pass (and every types-per-pass
+    // choice) draws the same near/far line and the maps are bit-identical
+    float smax = 0.f;
+    for (int t = 0; t < A.n_types; ++t) smax = fmaxf(smax, A.tlj[t].x);
+    s_bb[6] = smax;
+  }
   double ce = 0.0, cg0 = 0.0, cg1 = 0.0, cg2 = 0.0;
   float acc[TG][4];
 #pragma unroll
   for (int t = 0; t < TG; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = 0.f;
-  for (int base = 0; base < n_atoms; base += kTile) {
+  for (int base = 0; base < A.n_atoms; base += kTile) {
     __syncthreads();
     int a = base + threadIdx.x;
-    if (a < n_atoms) {
-      s_at[threadIdx.x] = atoms[a];
-      s_lj[threadIdx.x] = alj[a];
+    if (a < A.n_atoms) {
+      float4 at = A.atoms[a];
+      float2 lj = A.alj[a];
+      s_at[threadIdx.x] = at;
+      s_lj[threadIdx.x] = lj;
+      const float R = 0.5f * (lj.x + s_bb[6]);
+      const float r2 = R * R;
+      float dx = fmaxf(fmaxf(s_bb[0] - at.x, at.x - s_bb[3]), 0.f);
+      float dy = fmaxf(fmaxf(s_bb[1] - at.y, at.y - s_bb[4]), 0.f);
+      float dz = fmaxf(fmaxf(s_bb[2] - at.z, at.z - s_bb[5]), 0.f);
+      float keep = -1.f;
+      if (dx * dx + dy * dy + dz * dz < 1.01f * r2 + 1e-3f) {
+        int slot = atomicAdd(&s_nn, 1);
+        if (slot < kNearCap) {
+          s_nat[slot] = make_float4(at.x, at.y, at.z, r2);
+          s_nidx[slot] = a;
+          keep = r2;
+        }
+      }
+      s_r2[threadIdx.x] = keep;
     }
     __syncthreads();
-    int cnt = n_atoms - base < kTile ? n_atoms - base : kTile;
+    int cnt = A.n_atoms - base < kTile ? A.n_atoms - base : kTile;
     for (int u = 0; u < cnt; ++u) {
       float4 at = s_at[u];
       float2 lj = s_lj[u];
       float vx = at.x - px, vy = at.y - py, vz = at.z - pz;
-      float d2 = vx * vx + vy * vy + vz * vz;
+      float d2 = dist2_rn(vx, vy, vz);
+      const bool near = d2 < s_r2[u];  // summed in FP64 after the sweep
       float d = sqrtf(d2);
-      float inv_d = 1.f / (d + kEps);
+      float inv_d = near ? 0.f : 1.f / (d + kEps);
       float irsqr = 1.f / (d2 + kEps);
       if (with_coulomb) {
-        float e = at.w * inv_d * kCoulomb;
-        float de = -(at.w / (d2 * d + kEps)) * kCoulomb;
+        float q = near ? 0.f : at.w;
+        float e = q * inv_d * kCoulomb;
+        float de = -(q / (d2 * d + kEps)) * kCoulomb;
         ce += e;
         cg0 -= (double)(vx * de);
         cg1 -= (double)(vy * de);
@@ -93,16 +160,85 @@ __global__ __launch_bounds__(256) void k_grid(const float4* __restrict__ atoms,
       }
     }
   }
-  if (p >= P) return;
+  if (p >= A.P) return;
+  const int nn = s_nn < kNearCap ? s_nn : kNearCap;
+  const double qx = A.init64[0] + A.spacing64 * i, qy = A.init64[1] + A.spacing64 * j,
+               qz = A.init64[2] + A.spacing64 * k;
   const float big = 3.0e37f;  // FP32 maps saturate instead of overflowing
-  auto sat = [&](float v) { return fminf(fmaxf(v, -big), big); };
-  if (with_coulomb)
-    maps[p] = make_float4(sat((float)ce), sat((float)cg0), sat((float)cg1), sat((float)cg2));
+  auto sat = [&](double v) { return (float)fmin(fmax(v, -(double)big), (double)big); };
+  auto near_term = [&](int c, double* dv, double* d2o, double* dd) -> bool {
+    float4 na = s_nat[c];
+    float vx = na.x - px, vy = na.y - py, vz = na.z - pz;
+    if (!(dist2_rn(vx, vy, vz) < na.w)) return false;
+    double4 at = A.atoms64[s_nidx[c]];
+    dv[0] = at.x - qx;
+    dv[1] = at.y - qy;
+    dv[2] = at.z - qz;
+    *d2o = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+    *dd = sqrt(*d2o);
+    return true;
+  };
+  if (with_coulomb) {
+    for (int c = 0; c < nn; ++c) {
+      double v[3], d2, d;
+      if (!near_term(c, v, &d2, &d)) continue;
+      const double q = A.atoms64[s_nidx[c]].w, kc = 332.0637141491396;
+      ce += q / (d + 1e-20) * kc;
+      const double de = -(q / (d2 * d + 1e-20)) * kc;
+      cg0 -= v[0] * de;
+      cg1 -= v[1] * de;
+      cg2 -= v[2] * de;
+    }
+    A.maps[p] = make_float4(sat(ce), sat(cg0), sat(cg1), sat(cg2));
+  }
+  // the type maps: the FP32 sums of the sweep are stored first (which frees
+  // their registers), then every pair of maps is read back, completed with
+  // the FP64 near terms and stored again
 #pragma unroll
   for (int t = 0; t < TG; ++t)
     if (t < nt)
-      maps[(long long)(1 + type0 + t) * P + p] =
+      A.maps[(long long)(1 + type0 + t) * A.P + p] =
           make_float4(sat(acc[t][0]), sat(acc[t][1]), sat(acc[t][2]), sat(acc[t][3]));
+  if (nn == 0) return;
+  constexpr int kC = 2;  // maps completed per pass over the near list
+  for (int t0 = 0; t0 < nt; t0 += kC) {
+    double s4[kC][4];
+#pragma unroll
+    for (int t = 0; t < kC; ++t) {
+      float4 m = t0 + t < nt ? A.maps[(long long)(1 + type0 + t0 + t) * A.P + p]
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+      s4[t][0] = m.x;
+      s4[t][1] = m.y;
+      s4[t][2] = m.z;
+      s4[t][3] = m.w;
+    }
+    bool any = false;
+    for (int c = 0; c < nn; ++c) {
+      double v[3], d2, d;
+      if (!near_term(c, v, &d2, &d)) continue;
+      any = true;
+      const double2 alj = A.alj64[s_nidx[c]];
+      const double inv = 1.0 / (d + 1e-20), irs = 1.0 / (d2 + 1e-20);
+#pragma unroll
+      for (int t = 0; t < kC; ++t) {
+        const double2 tl = A.tlj64[type0 + (t0 + t < nt ? t0 + t : t0)];
+        const double uu = (tl.x + alj.x) * 0.5 * inv;
+        const double u2 = uu * uu, u6 = u2 * u2 * u2, u12 = u6 * u6;
+        const double e4 = tl.y * alj.y;
+        s4[t][0] += (u12 - u6) * e4;
+        const double de = (-12.0 * u12 + 6.0 * u6) * irs * e4;
+        s4[t][1] -= v[0] * de;
+        s4[t][2] -= v[1] * de;
+        s4[t][3] -= v[2] * de;
+      }
+    }
+    if (!any) return;  // no atom of the list is near this point
+#pragma unroll
+    for (int t = 0; t < kC; ++t)
+      if (t0 + t < nt)
+        A.maps[(long long)(1 + type0 + t0 + t) * A.P + p] =
+            make_float4(sat(s4[t][0]), sat(s4[t][1]), sat(s4[t][2]), sat(s4[t][3]));
+  }
 }
 
 template <int TG>
@@ -110,9 +246,7 @@ static void grid_passes(const GridBuildArgs& a, hipStream_t st) {
   const int threads = 256;
   long long blocks = (a.P + threads - 1) / threads;
   for (int t0 = 0, first = 1; t0 < a.n_types || first; t0 += TG, first = 0) {
-    hipLaunchKernelGGL(k_grid<TG>, dim3((unsigned)blocks), dim3(threads), 0, st, a.atoms, a.alj,
-                       a.n_atoms, a.tlj, a.n_types, t0, a.init[0], a.init[1], a.init[2], a.np[1],
-                       a.np[2], a.spacing, a.P, a.maps, first);
+    hipLaunchKernelGGL(k_grid<TG>, dim3((unsigned)blocks), dim3(threads), 0, st, a, t0, first);
   }
 }
 

@@ -13,12 +13,17 @@ constexpr int kLbfgsVpl = 7;      // vector elements per lane: 3*128 + 64 <= 64*
 struct GridBuildArgs {
   const float4* atoms;  // fixed receptor atoms x, y, z, q
   const float2* alj;    // sigma, sqrt(eps)
+  const double4* atoms64;  // the same in FP64 (terms of near atoms)
+  const double2* alj64;
   int n_atoms;
   const float2* tlj;    // per requested type: sigma, 4 * sqrt(eps)
+  const double2* tlj64;
   int n_types;
   float init[3];
+  double init64[3];
   int np[3];
   float spacing;
+  double spacing64;
   long long P;
   float4* maps;
 };

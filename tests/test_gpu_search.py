@@ -250,3 +250,71 @@ def test_gpu_search_distributions_over_seeds_match_oracle(sw):
               f"KS p(score) {p_s:.3g} p(rmsd) {p_r:.3g}")
         assert p_s > 1e-3 and p_r > 1e-3
         assert abs(gs.min() - os_.min()) <= 1.0
+
+
+def test_gpu_gaussian_walk_matches_oracle_without_optimiser(sw):
+    """The full-search mutation (pxd_mc_params.mutation 1: translation
+    N(0, 1 A^2) per axis, rotation vector N(0, 0.3^2 rad^2), one torsion
+    uniform), step by step against the oracle on the shared stream."""
+    prm = _no_opt(23, 20)
+    prm.mutation = 1
+    prm.trans_sigma, prm.rot_sigma = 1.0, 0.3
+    ok, n, _ = _compare_with_oracle(sw, prm, 8)
+    assert ok >= n - n // 8, (ok, n)
+
+
+def _search_params(seed, steps):
+    """The full docking search's chain parameters: Gaussian mutation, local
+    L-BFGS with history 8, at most 300 iterations, |g|inf <= 1e-3."""
+    prm = ffi.mc_params(seed=seed, max_nsteps=steps, steps_per_launch=5)
+    prm.mutation = 1
+    prm.trans_sigma, prm.rot_sigma = 1.0, 0.3
+    prm.relax = ffi.lbfgs_params(300, history=8, tol_grad=1e-3)
+    prm.full = ffi.lbfgs_params(300, history=8, tol_grad=1e-3)
+    return prm
+
+
+def test_gpu_full_search_parameters_match_oracle_distribution(sw):
+    """The full-search chain (Gaussian mutation, L-BFGS history 8 / 300
+    iterations / 1e-3) over 4 seeds x 6 walkers per ligand: the best pscores
+    and the evaluation counts per walker come from the oracle's distributions
+    (two-sample KS not rejecting at 0.1 %), the top-9 clustering equals the
+    oracle's pose_cluster on the device candidates."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from scipy.stats import ks_2samp
+    walkers, steps = 6, 6
+    gpu_ps, gpu_ne, jobs = {0: [], 1: []}, {0: [], 1: []}, []
+    for seed in range(31, 35):
+        prm = _search_params(seed, steps)
+        print(f"\n[mc] full-search parameters seed={seed}", flush=True)
+        res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], walkers,
+                            num_modes=9, min_rmsd=0.5)
+        bp, ne = res.best_pscore.cpu().numpy(), res.nevals.cpu().numpy()
+        ci, cc = res.cluster_idx.cpu().numpy(), res.cluster_count.cpu().numpy()
+        for l, lig in enumerate(sw["ligs"]):
+            for k in range(1, walkers):  # walker 0 takes no Monte-Carlo step
+                gpu_ps[l].append(float(bp[l * walkers + k]))
+                gpu_ne[l].append(int(ne[l * walkers + k]))
+                jobs.append((l, k, prm))
+            xs = np.stack([res.walker_xyz(l * walkers + k) for k in range(walkers)]).astype(np.float64)
+            out = np.zeros(9, np.int32)
+            cnt = O.lib().or_pose_cluster(walkers, lig.n_atoms, O.dp(np.ascontiguousarray(xs)),
+                                          O.dp(np.ascontiguousarray(bp[l * walkers:(l + 1) * walkers])),
+                                          O.ip(np.ascontiguousarray(ne[l * walkers:(l + 1) * walkers])),
+                                          9, 0.5, O.ip(out))
+            assert cc[l] == cnt
+            np.testing.assert_array_equal(ci[l, :cnt], out[:cnt])
+    with ThreadPoolExecutor(16) as ex:
+        outs = list(ex.map(lambda j: (j[0], O.mc_walker(sw["systems"][j[0]], sw["ligs"][j[0]],
+                                                        sw["sp"], j[2], sw["box"], j[1], j[0],
+                                                        sw["x0"][j[0]])), jobs))
+    for l in gpu_ps:
+        ops = np.array([o["pscore"] for m, o in outs if m == l])
+        one = np.array([o["nevals"] for m, o in outs if m == l])
+        gps, gne = np.array(gpu_ps[l]), np.array(gpu_ne[l])
+        p_s, p_n = ks_2samp(gps, ops).pvalue, ks_2samp(gne, one).pvalue
+        print(f"   ligand {l}: best pscore median gpu {np.median(gps):.3f} oracle "
+              f"{np.median(ops):.3f} (KS p {p_s:.3g}); evaluations median gpu {np.median(gne):.0f} "
+              f"oracle {np.median(one):.0f} (KS p {p_n:.3g})", flush=True)
+        assert p_s > 1e-3 and p_n > 1e-3
