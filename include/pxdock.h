@@ -379,6 +379,10 @@ typedef struct pxd_mc_params {
   int32_t steps_per_launch; /* steps per kernel launch of the captured step loop */
   pxd_lbfgs_params relax;   /* after each mutation: relax_nsteps (40) iterations */
   pxd_lbfgs_params full;    /* initial and refine: 240 iterations */
+  double mmenergy_threshold; /* > 0: a mutated pose whose energy exceeds the lowest
+                                pre-optimisation energy of its ligand's walkers by more
+                                is skipped (searcher.cpp:141-157); <= 0 (default -1,
+                                api.cpp:61): off */
 } pxd_mc_params;
 
 /* Reference defaults (bdock.cpp:60-78, searcher.h:33). */
@@ -403,6 +407,22 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* grid,
                          double* best_energy, double* best_pscore, int32_t* nevals,
                          int32_t num_modes, double min_rmsd, int32_t* cluster_idx,
                          int32_t* cluster_count, void* stream);
+
+/* pxd_mc_search with pre-generated conformers (searcher.cpp:107-113):
+ * ligand l brings conf_count[l] >= 1 conformations (HOST array), stored in
+ * the DEVICE array conformers ligand-major, [sum_l conf_count[l]*natoms_l][3].
+ * Walker 0 optimises conformer 0; walker k >= 1 is placed at random in the
+ * box from its own conformer k when k < conf_count[l] (keeping its
+ * torsions), else from conformer 0 with random torsions.  Everything else as
+ * pxd_mc_search, which is this call with one conformer per ligand. */
+pxd_status pxd_mc_search_conformers(const pxd_receptor* r, const pxd_grid* grid,
+                                    const pxd_ligand_batch* b, const pxd_score_params* sp,
+                                    const pxd_mc_params* prm, const pxd_box* box,
+                                    int32_t walkers, const int32_t* conf_count,
+                                    const float* conformers, float* best_xyz, float* best_tors,
+                                    double* best_energy, double* best_pscore, int32_t* nevals,
+                                    int32_t num_modes, double min_rmsd, int32_t* cluster_idx,
+                                    int32_t* cluster_count, void* stream);
 
 /* free_ligand::apply_parameters (data.cpp:592-641) for a batch of poses:
  * pose p of ligand pose_lig[p] is rebuilt from xyz_in (its input conformation)

@@ -1637,17 +1637,19 @@ static double mc_pscore(const or_system* s, const pxd_score_params* sp, const do
   return v;
 }
 
-/* randomize (searcher.cpp:46-87) from the input conformation xyz0. */
+/* randomize (searcher.cpp:46-87) from the input conformation xyz0.  The
+ * torsions are drawn only when xyz0 is the ligand's conformer 0
+ * (searcher.cpp:62-66); a walker's own pre-generated conformer keeps its. */
 static void mc_randomize(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
                          const pxd_mc_params* prm, const pxd_box* box, mc_rng* r,
-                         const double* xyz0, double* out) {
+                         const double* xyz0, int own_conformer, double* out) {
   int n = s->lig->n_atoms;
-  double tors[PXD_MAX_FRAGMENTS];
+  double tors[PXD_MAX_FRAGMENTS] = {0};
   for (int tries = 1;; ++tries) {
     double cen[3], q[4], rot[9];
     for (int d = 0; d < 3; ++d) cen[d] = rng_uniform(r, box->init_xyz[d], box->oppo_xyz[d]);
     rng_quaternion(r, q);
-    for (int k = 0; k < t->n_torsions; ++k)
+    for (int k = 0; !own_conformer && k < t->n_torsions; ++k)
       tors[k] = rng_uniform(r, -3.141592653589793, 3.141592653589793);
     or_rotmat_quat(q, rot);
     or_ligand_apply(t, n, xyz0, cen, rot, tors, out);
@@ -1741,8 +1743,9 @@ void or_mc_draws(int32_t kind, uint64_t seed, int32_t walker, int32_t ligand, in
  * walker.  Returns the number of Monte-Carlo steps taken. */
 int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
                      const pxd_mc_params* prm, const pxd_box* box, int32_t walker,
-                     int32_t ligand, const double* xyz0, double* best_xyz, double* best_tors,
-                     double* best_e, double* best_ps, int32_t* nevals) {
+                     int32_t ligand, const double* xyz0, int32_t own_conformer,
+                     double* emin_io, double* best_xyz, double* best_tors, double* best_e,
+                     double* best_ps, int32_t* nevals, int32_t* n_pruned) {
   const int n = s->lig->n_atoms, n3 = 3 * n;
   const int nt = s->leaves ? s->leaves->n_leaves : 0;
   mc_rng r = {{(uint32_t)prm->seed, (uint32_t)(prm->seed >> 32) + (uint32_t)walker},
@@ -1753,7 +1756,10 @@ int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score
   if (walker == 0)
     memcpy(cur, xyz0, sizeof(double) * n3);
   else
-    mc_randomize(s, t, sp, prm, box, &r, xyz0, cur);
+    mc_randomize(s, t, sp, prm, box, &r, xyz0, own_conformer, cur);
+  if (n_pruned) *n_pruned = 0;
+  double* pg = (double*)malloc(sizeof(double) * 3 * n);
+  double ptg[PXD_MAX_LEAVES + 1];
   double e, cur_ps;
   int32_t ne = 0;
   or_lbfgs(s, &prm->full, cur, ct, &e, &ne);
@@ -1770,6 +1776,16 @@ int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score
         memcpy(cdd, cur, sizeof(double) * n3);
         memcpy(dt, ct, sizeof(dt));
         mc_mutate(t, prm, &r, n, cdd);
+        /* optional prune on the ligand's lowest pre-optimisation energy
+         * (searcher.cpp:141-157; the minimum starts at 1e8, reader.cpp:51-53) */
+        if (prm->mmenergy_threshold > 0.0 && emin_io) {
+          double e0 = or_system_energy(s, dt, cdd, ptg, pg);
+          if (e0 < *emin_io) *emin_io = e0;
+          if (e0 > *emin_io + prm->mmenergy_threshold) {
+            if (n_pruned) ++*n_pruned;
+            continue;
+          }
+        }
         double ec, pc;
         or_lbfgs(s, &prm->relax, cdd, dt, &ec, &ne);
         *nevals += ne;
@@ -1799,5 +1815,6 @@ int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score
   }
   free(cur);
   free(cdd);
+  free(pg);
   return steps;
 }

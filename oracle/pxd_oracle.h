@@ -157,12 +157,18 @@ int32_t or_pose_cluster(int32_t n_cand, int32_t n_atoms, const double* xyz,
 /* ---- Monte-Carlo walker (searcher.cpp:46-223) ----
  * Walker `walker` of ligand `ligand` draws Philox4x32-10 blocks with key
  * (seed_lo, seed_hi + walker) and counter (draw, ligand, 0, 0); walker 0
- * optimises xyz0 and takes no Monte-Carlo step.  best_tors has n_leaves
- * entries.  Returns the number of Monte-Carlo steps taken. */
+ * optimises xyz0 and takes no Monte-Carlo step.  xyz0 is the conformer the
+ * walker starts from: its own pre-generated one (own_conformer 1: randomize
+ * draws no torsions) or the ligand's conformer 0 (searcher.cpp:107-113).
+ * emin_io: the ligand's lowest pre-optimisation energy, shared by its
+ * walkers (start at 1e8), used when prm->mmenergy_threshold > 0; may be
+ * NULL.  best_tors has n_leaves entries; n_pruned (may be NULL) receives the
+ * steps the prune skipped.  Returns the number of Monte-Carlo steps taken. */
 int32_t or_mc_walker(const or_system* s, const pxd_frag_tree* t, const pxd_score_params* sp,
                      const pxd_mc_params* prm, const pxd_box* box, int32_t walker,
-                     int32_t ligand, const double* xyz0, double* best_xyz, double* best_tors,
-                     double* best_e, double* best_ps, int32_t* nevals);
+                     int32_t ligand, const double* xyz0, int32_t own_conformer,
+                     double* emin_io, double* best_xyz, double* best_tors, double* best_e,
+                     double* best_ps, int32_t* nevals, int32_t* n_pruned);
 
 /* One mutation of the walker (mutate_inplace, searcher.cpp:189-223, or the
  * Gaussian variant of pxd_mc_params.mutation 1) applied to xyz in place,

@@ -77,7 +77,8 @@ def lib():
                                      C.POINTER(C.c_uint32)]),
             "or_pose_cluster": (i, [i, i, f64p, f64p, i32p, i, d, i32p]),
             "or_set_eval_noise": (None, [d, C.c_uint64]),
-            "or_mc_walker": (i, [P, P, P, P, P, i, i, f64p, f64p, f64p, f64p, f64p, i32p]),
+            "or_mc_walker": (i, [P, P, P, P, P, i, i, f64p, i, f64p, f64p, f64p, f64p, f64p,
+                                 i32p, i32p]),
             "or_dof_grad": (None, [P, i, f64p, f64p, f64p, f64p]),
             "or_dof_energy": (d, [P, f64p, f64p, f64p, f64p]),
             "or_mc_mutate": (i, [P, P, i, i, C.POINTER(C.c_uint32), i, f64p]),
@@ -294,19 +295,24 @@ def receptor_apply(leaves, xyz, tors):
     return out
 
 
-def mc_walker(system, ligand, sp, prm, box, walker, ligand_index, xyz0):
-    """One Monte-Carlo walker (searcher.cpp:98-187) with the GPU's Philox draws."""
+def mc_walker(system, ligand, sp, prm, box, walker, ligand_index, xyz0, own_conformer=False,
+              emin=None):
+    """One Monte-Carlo walker (searcher.cpp:98-187) with the GPU's Philox draws.
+    emin: a 1-element float64 array holding the ligand's lowest
+    pre-optimisation energy (updated in place), for the prune."""
     tc = ligand.tree.to_c()
     n = ligand.n_atoms
     bx = np.zeros((n, 3))
     bt = np.zeros(max(system.n_leaves, 1))
     be, bp = np.zeros(1), np.zeros(1)
     ne = np.zeros(1, np.int32)
+    npr = np.zeros(1, np.int32)
     steps = lib().or_mc_walker(byref(system.c), byref(tc), byref(sp), byref(prm), byref(box),
-                               walker, ligand_index, dp(f64(xyz0).reshape(-1, 3)), dp(bx),
-                               dp(bt), dp(be), dp(bp), ip(ne))
+                               walker, ligand_index, dp(f64(xyz0).reshape(-1, 3)),
+                               1 if own_conformer else 0, dp(emin) if emin is not None else None,
+                               dp(bx), dp(bt), dp(be), dp(bp), ip(ne), ip(npr))
     return dict(xyz=bx, tors=bt[:system.n_leaves], energy=float(be[0]), pscore=float(bp[0]),
-                nevals=int(ne[0]), steps=int(steps))
+                nevals=int(ne[0]), steps=int(steps), pruned=int(npr[0]))
 
 
 def mc_mutate(tree, prm, xyz, walker=1, ligand=0, ctr=0):

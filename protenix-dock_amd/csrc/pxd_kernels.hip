@@ -619,7 +619,7 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
 template <int VPL>
 __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy, float* hs,
                             double* hr, double* s_al, double* energy_out,
-                            bool* converged_out) {
+                            bool* converged_out, bool evaluate_only = false) {
   const int lane = lane_id();
   const int m = P.history;
   // history entries are stored at the class's full width (VPL elements per
@@ -637,7 +637,9 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
   double loss = eval_dir(c, 0.0, d, g);
   double energy = loss;  // kept unless a step is taken (optimizer.cpp:554-558)
   bool converged = false, updated = false;
-  if (vamax<VPL>(c.nk, g) < (float)P.tol_grad) {
+  if (evaluate_only) {
+    // the energy at x alone (no iteration, x unchanged)
+  } else if (vamax<VPL>(c.nk, g) < (float)P.tol_grad) {
     converged = true;
   } else {
     int head = 0, num_old = 0, n_iter = 0;
@@ -693,7 +695,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         // only waits for the one entry it needs.
         // The arithmetic does not depend on kPre or VPL: every size class
         // gives the same result for the same pose.
-        constexpr int kPre = VPL <= 2 ? 8 : (VPL <= 4 ? 6 : 2);
+        constexpr int kPre = VPL <= 2 ? PXD_LBFGS_PRE2 : (VPL <= 4 ? PXD_LBFGS_PRE4 : PXD_LBFGS_PRE7);
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
         double br[kPre];
 #pragma unroll
@@ -819,7 +821,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
 // Occupancy target 3 waves per SIMD (168 VGPRs, a few spilled): measured
 // 0.139 s against 0.145 s at the default 2 waves on the bench L-BFGS.
 template <int VPL, int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_lbfgs(DevReceptor R, DevLigands L,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves))) void k_lbfgs(DevReceptor R, DevLigands L,
                                               const DevGrid* __restrict__ G, LbfgsArgs A,
                                               int vpl_lo) {
   __shared__ float vpool[V_COUNT * kWave * VPL];
@@ -1070,6 +1072,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
   __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ double s_terms[PXD_NUM_TERMS];
   __shared__ int s_w;
+  __shared__ double s_emin;
   const int lane = lane_id();
   constexpr int NV = kWave * VPL;
   const McParamsDev& P = A.prm;
@@ -1122,9 +1125,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
     for (int k = lane; k < nl; k += kWave) curt[k] = A.cur_tors[(long long)w * nl + k];
     int phase;
     if (st.mode == 0) {
-      // initial conformer (searcher.cpp:105-115): walker 0 keeps pose 0,
-      // the others are randomised (searcher.cpp:46-87)
-      for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = A.lig_xyz0[3ll * li.atom_off + k];
+      // initial conformer (searcher.cpp:105-115): walker 0 keeps conformer
+      // 0; walker pid starts from its own pre-generated conformer pid when
+      // there is one (keeping its torsions), else from conformer 0 with
+      // random torsions; both are placed at random (searcher.cpp:46-87)
+      const int lig = A.walker_lig[w];
+      const int src = pid > 0 && pid < A.lig_conf_n[lig] ? pid : 0;
+      const float* x0 = A.lig_xyz0 + 3ll * (A.lig_conf_off[lig] + (long long)src * n);
+      for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = x0[k];
       __syncthreads();
       if (pid != 0) {
         for (int tries = 1;; ++tries) {
@@ -1137,12 +1145,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
           for (int f = lane; f < li.n_frags; f += kWave) ap.th[f] = 0.f;
           __syncthreads();
           // torsion draws are wave-uniform: every lane draws, lane 0 stores
-          for (int t = 0; t < li.n_tors; ++t) {
+          for (int t = 0; src == 0 && t < li.n_tors; ++t) {
             float v = (float)rng.uniform(-3.141592653589793, 3.141592653589793);
             if (lane == 0) ap.th[L.tors_frag[li.frag_off + t]] = v;
           }
           quat_to_rot(q, rot);
-          for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = A.lig_xyz0[3ll * li.atom_off + k];
+          for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = x0[k];
           __syncthreads();
           apply_ligand(L, li, c.s.xyz, cen, rot, ap);
           bool inbox = all_in_box(c.s.xyz, n, box);
@@ -1161,8 +1169,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
       phase = PH_MUTATE;
     }
     const int step_end = A.step_end < P.max_nsteps ? A.step_end : P.max_nsteps;
+    // the energy prune (searcher.cpp:141-157) evaluates the mutated pose
+    // before its relax: that is the L-BFGS site below asked for its initial
+    // evaluation only (not counted in nevals, as the reference's), so the
+    // kernel keeps a single inlined evaluation
+    bool probe = false, mutated = false;
     for (;;) {
-      if (phase == PH_MUTATE) {
+      if (phase == PH_MUTATE && mutated) {
+        mutated = false;  // the pose passed the prune: relax it
+      } else if (phase == PH_MUTATE) {
         if (st.step >= step_end) break;
         // mutate_inplace (searcher.cpp:189-223) on a copy of the current state
         for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = curx[k];
@@ -1216,16 +1231,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
         if (apply) apply_ligand(L, li, c.s.xyz, cen, rot, ap);
         for (int k = lane; k < n3; k += kWave) x[k] = c.s.xyz[k];
         for (int k = lane; k < nl; k += kWave) x[n3 + k] = curt[k];
+        probe = P.mm_threshold > 0.0;
       } else if (phase == PH_REFINE) {
         for (int k = lane; k < n3; k += kWave) x[k] = curx[k];
         for (int k = lane; k < nl; k += kWave) x[n3 + k] = curt[k];
       }
       __syncthreads();
-      // the single L-BFGS site: full step for init / refine, relax after a mutation
+      // the single L-BFGS site: full step for init / refine, relax after a
+      // mutation, no iteration for the prune's evaluation
       c.nevals = 0;
       double e;
       bool conv;
-      lbfgs_solve<VPL>(c, phase == PH_MUTATE ? P.relax : P.full, hy, hs, hr, s_al, &e, &conv);
+      lbfgs_solve<VPL>(c, phase == PH_MUTATE ? P.relax : P.full, hy, hs, hr, s_al, &e, &conv,
+                       probe);
+      if (probe) {
+        // the mutated pose's energy updates the ligand's lowest
+        // pre-optimisation energy, shared by its walkers (starts at 1e8); a
+        // pose above it by more than the threshold skips this step: no
+        // relax and no Metropolis draw
+        probe = false;
+        if (lane == 0) {
+          unsigned long long* slot = A.lig_emin + A.walker_lig[w];
+          unsigned long long old = e == e ? atomicMin(slot, order_key(e)) : *slot;
+          double m = order_key_value(old);
+          s_emin = e < m ? e : m;
+        }
+        __syncthreads();
+        const double emin = s_emin;
+        __syncthreads();
+        if (e > emin + P.mm_threshold) {
+          st.step++;
+        } else {
+          mutated = true;  // x still holds the mutated pose
+        }
+        continue;
+      }
       // the single pscore site, at the optimised pose
       for (int k = lane; k < n3; k += kWave) c.s.xyz[k] = x[k];
       for (int k = lane; k < nl; k += kWave) c.s.tors[k] = x[n3 + k];
@@ -1316,6 +1356,16 @@ void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, cons
                      kLbfgsClassVpl[0]);
   c.counter = a.counter + 2;
   hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[0]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c, 0);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {

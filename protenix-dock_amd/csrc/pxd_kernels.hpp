@@ -75,6 +75,22 @@ constexpr int kLbfgsClassVpl[kLbfgsClasses] = {2, 4, kLbfgsVpl};
 
 // Occupancy target of the Monte-Carlo walker kernel (waves per SIMD); the
 // search launches this many walker slots per SIMD.
+// Occupancy target of the batched L-BFGS kernel (waves per SIMD) and the
+// depth of its two-loop history prefetch per size class (vector width 2 / 4 / 7).
+#ifndef PXD_LBFGS_WAVES
+#define PXD_LBFGS_WAVES 3
+#endif
+#ifndef PXD_LBFGS_PRE2
+#define PXD_LBFGS_PRE2 8
+#endif
+#ifndef PXD_LBFGS_PRE4
+#define PXD_LBFGS_PRE4 6
+#endif
+#ifndef PXD_LBFGS_PRE7
+#define PXD_LBFGS_PRE7 2
+#endif
+constexpr int kLbfgsWaves = PXD_LBFGS_WAVES;
+
 #ifndef PXD_MC_WAVES
 #define PXD_MC_WAVES 2
 #endif
@@ -98,6 +114,7 @@ struct McParamsDev {
   float amplitude, temperature, trans_sigma, rot_sigma;
   int max_tries;
   LbfgsParamsDev relax, full;
+  double mm_threshold;  // > 0: the energy prune of searcher.cpp:141-157
 };
 
 // Per-walker scalars kept on the device between launches.
@@ -117,7 +134,10 @@ struct McArgs {
   const int* walker_lig;
   const int* walker_pid;   // walker index within its ligand (0: no Monte-Carlo)
   const int* walker_off;   // atom offset of the walker's conformations
-  const float* lig_xyz0;   // input conformation, per ligand atom offset
+  const float* lig_xyz0;   // conformers: ligand l's k-th at atom lig_conf_off[l] + k * natoms
+  const int* lig_conf_off; // [n_ligands]
+  const int* lig_conf_n;   // [n_ligands] conformers of each ligand (>= 1)
+  unsigned long long* lig_emin;  // [n_ligands] lowest pre-optimisation energy (ordered key)
   float* cur_xyz;
   float* best_xyz;
   float* cur_tors;
@@ -157,6 +177,21 @@ void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_sco
 void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
                int n_slots, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
+void launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
+
+// A total order on doubles as unsigned 64-bit keys (a < b iff key(a) < key(b)
+// for non-NaN values), so a minimum over walkers is one atomicMin.
+__host__ __device__ inline unsigned long long order_key(double v) {
+  unsigned long long b;
+  __builtin_memcpy(&b, &v, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double order_key_value(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double v;
+  __builtin_memcpy(&v, &b, 8);
+  return v;
+}
 void launch_cluster(const ClusterArgs& a, int n_atoms_max, hipStream_t st);
 struct ApplyArgs {
   int n_poses;

@@ -43,6 +43,7 @@ extern "C" {
 
 void pxd_mc_default_params(pxd_mc_params* p) {
   memset(p, 0, sizeof(*p));
+  p->mmenergy_threshold = -1.0;
   p->seed = 0;
   p->max_nsteps = 200;
   p->mutation = 0;
@@ -56,15 +57,25 @@ void pxd_mc_default_params(pxd_mc_params* p) {
   pxd_lbfgs_default_params(&p->full, 240);
 }
 
-pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
-                         const pxd_score_params* sp, const pxd_mc_params* prm, const pxd_box* box,
-                         int32_t walkers, const float* lig_xyz0, float* best_xyz,
-                         float* best_tors, double* best_energy, double* best_pscore,
-                         int32_t* nevals, int32_t num_modes, double min_rmsd,
-                         int32_t* cluster_idx, int32_t* cluster_count, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// pxd_mc_search / pxd_mc_search_conformers; conf_count == nullptr: one
+// conformer per ligand.
+pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                     const pxd_score_params* sp, const pxd_mc_params* prm, const pxd_box* box,
+                     int32_t walkers, const int32_t* conf_count, const float* lig_xyz0,
+                     float* best_xyz, float* best_tors, double* best_energy,
+                     double* best_pscore, int32_t* nevals, int32_t num_modes, double min_rmsd,
+                     int32_t* cluster_idx, int32_t* cluster_count, void* stream) {
   if (!r || !b || !sp || !prm || !box || !lig_xyz0 || !best_xyz || !best_energy ||
       !best_pscore || !nevals)
     return pxd_set_error_(PXD_ERR_ARG, "mc_search: null argument");
+  if (conf_count)
+    for (int l = 0; l < b->n; ++l)
+      if (conf_count[l] < 1)
+        return pxd_set_error_(PXD_ERR_ARG, "mc_search: every ligand needs a conformer");
   if (walkers < 1) return pxd_set_error_(PXD_ERR_ARG, "mc_search: walkers < 1");
   if (r->n_leaves > 0 && !best_tors) return pxd_set_error_(PXD_ERR_ARG, "mc_search: best_tors required");
   if (bad_lbfgs(prm->relax) || bad_lbfgs(prm->full))
@@ -103,6 +114,12 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   }
   lwoff[nl] = W;
   woff[W] = (int)atoms;
+  std::vector<int> cnum(nl), coff(nl);
+  for (int l = 0, a = 0; l < nl; ++l) {
+    cnum[l] = conf_count ? conf_count[l] : 1;
+    coff[l] = a;
+    a += cnum[l] * b->h_natoms[l];
+  }
   if (atoms > (1ll << 30)) return pxd_set_error_(PXD_ERR_RANGE, "mc_search: too many atoms");
   int dev = 0, ncu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
@@ -119,7 +136,8 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   hipGraphExec_t exec = nullptr;
   char* mem = nullptr;
   pxd_score_params* d_sp = nullptr;
-  int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt;
+  int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt, *d_coff, *d_cnum;
+  unsigned long long* d_emin;
   float *d_cur, *d_curt, *d_hist;
   WalkerState* d_st;
   size_t off = 0;
@@ -132,7 +150,8 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
                o_woff = take(4 * (W + 1)), o_lwoff = take(4 * (nl + 1)),
                o_cnt = take(4 * (size_t)n_launch * kLbfgsClasses), o_cur = take(12 * (size_t)atoms),
                o_curt = take(4 * (size_t)W * (L + 1)), o_st = take(sizeof(WalkerState) * W),
-               o_hist = take(4 * hist_floats);
+               o_hist = take(4 * hist_floats), o_coff = take(4 * nl), o_cnum = take(4 * nl),
+               o_emin = take(8 * (size_t)nl);
   HIP_OK(hipMalloc(&mem, off));
   d_sp = reinterpret_cast<pxd_score_params*>(mem + o_sp);
   d_wlig = reinterpret_cast<int*>(mem + o_wlig);
@@ -144,11 +163,16 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   d_curt = reinterpret_cast<float*>(mem + o_curt);
   d_st = reinterpret_cast<WalkerState*>(mem + o_st);
   d_hist = reinterpret_cast<float*>(mem + o_hist);
+  d_coff = reinterpret_cast<int*>(mem + o_coff);
+  d_cnum = reinterpret_cast<int*>(mem + o_cnum);
+  d_emin = reinterpret_cast<unsigned long long*>(mem + o_emin);
   HIP_OK(hipMemcpy(d_sp, sp, sizeof(*sp), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_wlig, wlig.data(), 4 * W, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_wpid, wpid.data(), 4 * W, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_woff, woff.data(), 4 * (W + 1), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_lwoff, lwoff.data(), 4 * (nl + 1), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_coff, coff.data(), 4 * nl, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_cnum, cnum.data(), 4 * nl, hipMemcpyHostToDevice));
   HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
@@ -168,6 +192,7 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
     a.prm.max_tries = prm->max_tries;
     a.prm.relax = to_dev(prm->relax);
     a.prm.full = to_dev(prm->full);
+    a.prm.mm_threshold = prm->mmenergy_threshold;
     a.sp = d_sp;
     for (int d = 0; d < 3; ++d) {
       a.box_lo[d] = (float)box->init_xyz[d];
@@ -178,6 +203,9 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
     a.walker_pid = d_wpid;
     a.walker_off = d_woff;
     a.lig_xyz0 = lig_xyz0;
+    a.lig_conf_off = d_coff;
+    a.lig_conf_n = d_cnum;
+    a.lig_emin = d_emin;
     a.cur_xyz = d_cur;
     a.best_xyz = best_xyz;
     a.cur_tors = d_curt;
@@ -192,6 +220,8 @@ pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
     HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
     launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, cs);
+    // the ligands' lowest pre-optimisation energies start at 1e8 (reader.cpp:51-53)
+    launch_fill_u64(d_emin, nl, order_key(1e8), cs);
     for (int k = 0; k < n_launch; ++k) {
       a.counter = d_cnt + k * kLbfgsClasses;
       a.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
@@ -237,6 +267,35 @@ done:
   if (mem) (void)hipFree(mem);
   if (err != hipSuccess) return pxd_set_error_(PXD_ERR_HIP, hipGetErrorString(err));
   return PXD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pxd_status pxd_mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_batch* b,
+                         const pxd_score_params* sp, const pxd_mc_params* prm, const pxd_box* box,
+                         int32_t walkers, const float* lig_xyz0, float* best_xyz,
+                         float* best_tors, double* best_energy, double* best_pscore,
+                         int32_t* nevals, int32_t num_modes, double min_rmsd,
+                         int32_t* cluster_idx, int32_t* cluster_count, void* stream) {
+  return mc_search(r, g, b, sp, prm, box, walkers, nullptr, lig_xyz0, best_xyz, best_tors,
+                   best_energy, best_pscore, nevals, num_modes, min_rmsd, cluster_idx,
+                   cluster_count, stream);
+}
+
+pxd_status pxd_mc_search_conformers(const pxd_receptor* r, const pxd_grid* g,
+                                    const pxd_ligand_batch* b, const pxd_score_params* sp,
+                                    const pxd_mc_params* prm, const pxd_box* box,
+                                    int32_t walkers, const int32_t* conf_count,
+                                    const float* conformers, float* best_xyz, float* best_tors,
+                                    double* best_energy, double* best_pscore, int32_t* nevals,
+                                    int32_t num_modes, double min_rmsd, int32_t* cluster_idx,
+                                    int32_t* cluster_count, void* stream) {
+  if (!conf_count) return pxd_set_error_(PXD_ERR_ARG, "mc_search_conformers: null conf_count");
+  return mc_search(r, g, b, sp, prm, box, walkers, conf_count, conformers, best_xyz, best_tors,
+                   best_energy, best_pscore, nevals, num_modes, min_rmsd, cluster_idx,
+                   cluster_count, stream);
 }
 
 pxd_status pxd_apply_poses(const pxd_ligand_batch* b, int32_t n_poses, const int32_t* pose_lig,

@@ -134,6 +134,7 @@ class PxdMcParams(C.Structure):
         ("amplitude", C.c_double), ("temperature", C.c_double), ("trans_sigma", C.c_double),
         ("rot_sigma", C.c_double), ("max_tries", C.c_int32), ("steps_per_launch", C.c_int32),
         ("relax", PxdLbfgsParams), ("full", PxdLbfgsParams),
+        ("mmenergy_threshold", C.c_double),
     ]
 
 
@@ -147,6 +148,7 @@ def mc_params(seed=0, max_nsteps=200, relax_iters=40, full_iters=240, steps_per_
     p.temperature = 1.2
     p.trans_sigma = 1.0
     p.rot_sigma = 0.3
+    p.mmenergy_threshold = -1.0
     p.max_tries = 2000
     p.steps_per_launch = steps_per_launch
     p.relax = lbfgs_params(relax_iters)

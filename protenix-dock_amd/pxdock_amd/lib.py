@@ -73,6 +73,8 @@ def load():
         "pxd_set_tuning": (i, [P]),
         "pxd_autotune": (i, [P, P, i32p, i, P, i, P, P, P, P, P, P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
+        "pxd_mc_search_conformers": (i, [P, P, P, P, P, P, i, i32p, P, P, P, P, P, P, i, d, P,
+                                         P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
@@ -348,13 +350,22 @@ class SearchResult:
 
 
 def mc_search(receptor, grid, batch, lig_xyz0, score_params, params, box, walkers,
-              num_modes=0, min_rmsd=1.0):
+              num_modes=0, min_rmsd=1.0, conformers=None):
     """pxd_mc_search: `walkers` Monte-Carlo walkers per ligand of `batch`.
-    lig_xyz0: list of [natoms, 3] input conformations, one per ligand."""
+    lig_xyz0: list of [natoms, 3] input conformations, one per ligand; or,
+    with conformers (pxd_mc_search_conformers), conformers[l] is ligand l's
+    list of [natoms, 3] conformations (lig_xyz0 is then ignored)."""
     import torch
     dev = torch.device("cuda", torch.cuda.current_device())
-    flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in lig_xyz0])
-    assert len(flat) == int(batch.natoms.sum()), "input conformations do not match the batch"
+    if conformers is not None:
+        counts = np.array([len(c) for c in conformers], np.int32)
+        flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3)
+                               for c in conformers for x in c])
+        assert len(flat) == int((counts * batch.natoms).sum()), "conformers do not match"
+    else:
+        counts = None
+        flat = np.concatenate([np.asarray(x, np.float64).reshape(-1, 3) for x in lig_xyz0])
+        assert len(flat) == int(batch.natoms.sum()), "input conformations do not match the batch"
     x0 = torch.from_numpy(flat.astype(np.float32).ravel()).to(dev)
     W = len(batch.natoms) * walkers
     nl = receptor.n_leaves
@@ -365,10 +376,17 @@ def mc_search(receptor, grid, batch, lig_xyz0, score_params, params, box, walker
     ne = torch.zeros(W, dtype=torch.int32, device=dev)
     ci = torch.full((len(batch.natoms), max(num_modes, 1)), -1, dtype=torch.int32, device=dev)
     cc = torch.zeros(len(batch.natoms), dtype=torch.int32, device=dev)
-    check(load().pxd_mc_search(receptor.h, grid.h if grid is not None else None, batch.h,
-                               _ref(score_params), _ref(params), _ref(box), walkers, _dev(x0),
-                               _dev(bx), _dev(bt), _dev(be), _dev(bp), _dev(ne), num_modes,
-                               min_rmsd, _dev(ci), _dev(cc), _stream()))
+    g = grid.h if grid is not None else None
+    if counts is None:
+        check(load().pxd_mc_search(receptor.h, g, batch.h, _ref(score_params), _ref(params),
+                                   _ref(box), walkers, _dev(x0), _dev(bx), _dev(bt), _dev(be),
+                                   _dev(bp), _dev(ne), num_modes, min_rmsd, _dev(ci), _dev(cc),
+                                   _stream()))
+    else:
+        check(load().pxd_mc_search_conformers(
+            receptor.h, g, batch.h, _ref(score_params), _ref(params), _ref(box), walkers,
+            counts.ctypes.data_as(i32p), _dev(x0), _dev(bx), _dev(bt), _dev(be), _dev(bp),
+            _dev(ne), num_modes, min_rmsd, _dev(ci), _dev(cc), _stream()))
     return SearchResult(batch, walkers, bx, bt, be, bp, ne, ci, cc)
 
 

@@ -318,3 +318,63 @@ def test_gpu_full_search_parameters_match_oracle_distribution(sw):
               f"{np.median(ops):.3f} (KS p {p_s:.3g}); evaluations median gpu {np.median(gne):.0f} "
               f"oracle {np.median(one):.0f} (KS p {p_n:.3g})", flush=True)
         assert p_s > 1e-3 and p_n > 1e-3
+
+
+def test_gpu_energy_prune_matches_oracle_without_optimiser(sw):
+    """searcher.cpp:141-157 on the device: two walkers per ligand (walker 0
+    takes no Monte-Carlo step, so the ligand minimum is walker 1's alone and
+    the walk is deterministic), optimiser off, step by step against the
+    oracle; the threshold is low enough for the oracle to prune steps."""
+    prm = _no_opt(41, 24)
+    prm.mmenergy_threshold = 25.0
+    res = lib.mc_search(sw["R"], sw["G"], sw["B"], sw["x0"], sw["sp"], prm, sw["box"], 2)
+    bp, ne = res.best_pscore.cpu().numpy(), res.nevals.cpu().numpy()
+    pruned, ok = 0, 0
+    for l, lig in enumerate(sw["ligs"]):
+        emin = np.array([1e8])
+        o = O.mc_walker(sw["systems"][l], lig, sw["sp"], prm, sw["box"], 1, l, sw["x0"][l],
+                        emin=emin)
+        w = 2 * l + 1
+        dx = float(np.abs(res.walker_xyz(w) - o["xyz"]).max())
+        print(f"\n[mc prune] ligand {l}: pruned {o['pruned']} of 24; pscore gpu {bp[w]:.4f} "
+              f"oracle {o['pscore']:.4f}; evaluations {ne[w]} / {o['nevals']}; dx {dx:.2e}",
+              flush=True)
+        pruned += o["pruned"]
+        ok += dx <= 2e-3 and ne[w] == o["nevals"] and \
+            abs(bp[w] - o["pscore"]) <= 1e-4 * max(abs(o["pscore"]), 1.0)
+    assert pruned > 0
+    assert ok == len(sw["ligs"])
+
+
+def test_gpu_own_conformer_starts_match_oracle_without_optimiser(sw):
+    """searcher.cpp:107-113: ligand l brings three conformers; walkers 1 and 2
+    start from their own (no torsion draws), walkers 3 and 4 from conformer
+    0 with random torsions; every walker against the oracle started the same
+    way, optimiser off."""
+    rng = np.random.default_rng(6)
+    confs = []
+    for l, lig in enumerate(sw["ligs"]):
+        cl = [sw["x0"][l]]
+        for _ in range(2):
+            t = rng.uniform(-np.pi, np.pi, lig.tree.n_torsions)
+            x = synth.ligand_apply(lig.tree, sw["x0"][l], sw["rec"].pocket_center, np.eye(3), t)
+            cl.append(np.asarray(np.asarray(x, np.float32), np.float64))
+        confs.append(cl)
+    walkers = 5
+    prm = _no_opt(43, 6)
+    res = lib.mc_search(sw["R"], sw["G"], sw["B"], None, sw["sp"], prm, sw["box"], walkers,
+                        conformers=confs)
+    bp, ne = res.best_pscore.cpu().numpy(), res.nevals.cpu().numpy()
+    ok, n = 0, 0
+    for l, lig in enumerate(sw["ligs"]):
+        for k in range(walkers):
+            own = 0 < k < len(confs[l])
+            o = O.mc_walker(sw["systems"][l], lig, sw["sp"], prm, sw["box"], k, l,
+                            confs[l][k] if own else confs[l][0], own_conformer=own)
+            w = l * walkers + k
+            dx = float(np.abs(res.walker_xyz(w) - o["xyz"]).max())
+            n += 1
+            ok += dx <= 2e-3 and ne[w] == o["nevals"] and \
+                abs(bp[w] - o["pscore"]) <= 1e-4 * max(abs(o["pscore"]), 1.0)
+    print(f"\n[mc conformers] agree {ok}/{n}", flush=True)
+    assert ok >= n - n // 8, (ok, n)

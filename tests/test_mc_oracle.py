@@ -153,3 +153,70 @@ def test_full_search_improves_on_the_start(mcw):
         rx = O.receptor_apply(mcw["rec"].leaves, mcw["rec"].xyz, out["tors"])
         ps, _ = O.pscore(mcw["sp"], mcw["rec"], rx, mcw["lig"], out["xyz"])
         assert out["pscore"] == pytest.approx(ps, rel=1e-12, abs=1e-12)
+
+
+# ---------------------------------------------------------------- energy prune and conformers
+
+def _prune(seed, steps, thr):
+    prm = _no_opt(seed, steps)
+    prm.mmenergy_threshold = thr
+    return prm
+
+
+def test_prune_with_a_loose_threshold_changes_nothing(mcw):
+    """The prune evaluates the mutated pose but draws nothing: with a
+    threshold nothing exceeds, the walk is the walk without the prune."""
+    a = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(31, 12), mcw["box"], 1, 0, mcw["x0"])
+    emin = np.array([1e8])
+    b = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _prune(31, 12, math.inf), mcw["box"], 1, 0,
+                    mcw["x0"], emin=emin)
+    assert b["pruned"] == 0
+    assert a["pscore"] == b["pscore"] and a["nevals"] == b["nevals"]
+    np.testing.assert_array_equal(a["xyz"], b["xyz"])
+    # the minimum tracked the mutated poses' energies
+    assert emin[0] < 1e8
+
+
+def test_prune_below_every_energy_skips_every_step(mcw):
+    """A ligand minimum far below any pose: every mutated pose exceeds it by
+    more than the threshold, so no step relaxes or moves the walker."""
+    first = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _no_opt(32, 0), mcw["box"], 1, 0,
+                        mcw["x0"])
+    emin = np.array([-1e9])
+    out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _prune(32, 9, 1.0), mcw["box"], 1, 0,
+                      mcw["x0"], emin=emin)
+    assert out["pruned"] == 9 and out["steps"] == 9
+    assert out["nevals"] == first["nevals"] and out["pscore"] == first["pscore"]
+    assert emin[0] == -1e9
+
+
+def test_prune_minimum_is_shared_and_bounds_the_walks(mcw):
+    """With the optimiser off, every accepted pose's energy was a pruning
+    candidate, so the shared minimum never exceeds a walker's best energy;
+    walkers of one ligand keep lowering the same minimum."""
+    emin = np.array([1e8])
+    seen = []
+    for w in (1, 2, 3):
+        out = O.mc_walker(mcw["s"], mcw["lig"], mcw["sp"], _prune(33, 15, 30.0), mcw["box"], w, 0,
+                          mcw["x0"], emin=emin)
+        seen.append(float(emin[0]))
+        assert emin[0] <= out["energy"] + 1e-9
+    assert seen[0] >= seen[1] >= seen[2]
+
+
+def test_own_conformer_start_keeps_its_torsions(mcw):
+    """searcher.cpp:107-113 / 62-66: a walker starting from its own
+    pre-generated conformer is only moved rigidly (every interatomic distance
+    kept); from conformer 0 it also gets random torsions."""
+    lig = mcw["lig"]
+    rng = np.random.default_rng(3)
+    conf = np.asarray(synth.ligand_apply(lig.tree, lig.xyz[0], mcw["rec"].pocket_center, np.eye(3),
+                                         rng.uniform(-np.pi, np.pi, lig.tree.n_torsions)))
+    d0 = np.linalg.norm(conf[:, None] - conf[None], axis=-1)
+    own = O.mc_walker(mcw["s"], lig, mcw["sp"], _no_opt(34, 0), mcw["box"], 2, 0, conf,
+                      own_conformer=True)
+    x = own["xyz"]
+    np.testing.assert_allclose(np.linalg.norm(x[:, None] - x[None], axis=-1), d0, atol=1e-9)
+    moved = O.mc_walker(mcw["s"], lig, mcw["sp"], _no_opt(34, 0), mcw["box"], 2, 0, conf)
+    y = moved["xyz"]
+    assert np.abs(np.linalg.norm(y[:, None] - y[None], axis=-1) - d0).max() > 0.1
