@@ -43,7 +43,8 @@ typedef enum pxd_status {
   PXD_ERR_CONF = 2,  /* malformed input data (failed_conf_error in the reference) */
   PXD_ERR_HIP = 3,   /* HIP runtime error */
   PXD_ERR_STATE = 4, /* call order misuse (e.g. no maps built) */
-  PXD_ERR_RANGE = 5  /* a size exceeds a documented maximum */
+  PXD_ERR_RANGE = 5, /* a size exceeds a documented maximum */
+  PXD_ERR_IO = 6     /* a file is missing, unreadable or truncated (file_system_error) */
 } pxd_status;
 
 /* Force-field and interaction-set data of one molecule.
@@ -283,6 +284,94 @@ pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types);
 pxd_status pxd_grid_set_layout(pxd_grid* g, int32_t layout, void* stream);
 pxd_status pxd_grid_set_slope(pxd_grid* g, double slope);
 void pxd_grid_destroy(pxd_grid* g);
+
+/* ------------------------------------------------------------------------ */
+/* Map directories (receptor_cache::save / restore, grid.cpp:305-431)        */
+/*
+ * A map directory holds one file per map and a metadata file written last,
+ * whose presence marks the directory complete (the role of meta.dat in
+ * grid.cpp:336-343).  The reference's Boost archives of FP64 maps are not
+ * read or written; the format is this library's own:
+ *
+ *   <tag>.pxm  one per GAFF2 type, the tag of pxdock_vdw_table.h ("c3",
+ *              "n+", ...); coul.pxm for the unit-probe Coulomb map (the tag
+ *              list of constant.h kVdwTypeTags, "coul" last).  A 64-byte
+ *              little-endian header
+ *                0  char[8]  "PXDMAP\0\1"
+ *                8  uint32   format version (PXD_MAPS_VERSION)
+ *               12  int32    GAFF2 type index, -1 for Coulomb
+ *               16  int32[3] npoints (x, y, z)
+ *               28  int32    values per point (4)
+ *               32  uint64   payload bytes = npoints product * 16
+ *               40  uint64   payload checksum (FNV-1a 64 over little-endian
+ *                            64-bit words: h = 0xcbf29ce484222325, per word
+ *                            h ^= w, h *= 0x100000001b3)
+ *               48  zero padding
+ *              then FP32 (value, dE/dx, dE/dy, dE/dz) per point, point
+ *              (i, j, k) at (i * ny + j) * nz + k (grid.h:120-122).
+ *   meta.pxd   text, one record per line:
+ *                pxdock-maps <version>
+ *                init_xyz <x> <y> <z>        (%.17g, exact for FP64)
+ *                npoints <nx> <ny> <nz>
+ *                spacing <h>
+ *                slope <s>
+ *                receptor <n_atoms> <key, 16 hex digits>
+ *                excluded <n> <atom> ...     (movable receptor atoms, ascending)
+ *                types <n> <type> ...        (ascending GAFF2 indices)
+ *                map <type|coul> <file> <checksum, 16 hex digits>   (one per map)
+ *                end
+ *
+ * Saving removes an old meta.pxd first and renames meta.pxd.tmp into place
+ * only after every map file is complete.  Loading fails with PXD_ERR_IO when
+ * meta.pxd or a map file is missing, unreadable, truncated or fails its
+ * checksum, and with PXD_ERR_CONF on a version mismatch (grid.cpp:356-363),
+ * malformed metadata, or maps saved for another receptor or another set of
+ * movable atoms. */
+#define PXD_MAPS_VERSION 1
+
+/* GAFF2 tag of type t (0..98, kVdwTypeTags, constant.h:131-231), "coul" for
+ * t = PXD_NUM_VDW_TYPES, NULL otherwise. */
+const char* pxd_vdw_type_tag(int32_t t);
+
+typedef struct pxd_maps_meta {
+  pxd_box box;
+  double slope;
+  int32_t n_atoms;        /* receptor atoms, movable ones included */
+  int32_t n_excluded;     /* movable receptor atoms, left out of the maps */
+  uint64_t receptor_key;  /* pxd_receptor_key of the receptor the maps belong to */
+  int32_t n_types;
+  int32_t types[PXD_NUM_VDW_TYPES]; /* ascending GAFF2 indices */
+} pxd_maps_meta;
+
+/* Fingerprint of what the maps depend on: FNV-1a 64 (as above, over bytes)
+ * of n_atoms and, for every receptor atom outside the leaves in index order,
+ * its index, x, y, z, charge, sigma and epsilon (int32 / FP64 little-endian).
+ * Host only; no device is touched. */
+pxd_status pxd_receptor_key(const pxd_mol* mol, const double* xyz, const pxd_leaves* leaves,
+                            uint64_t* key);
+/* Host-only directory I/O.  maps is FP32 [(n_types+1)][npoints][4] in the
+ * layout of pxd_grid_upload: map 0 = Coulomb, map t+1 = meta->types[t].
+ * excluded holds meta->n_excluded ascending atom indices. */
+pxd_status pxd_maps_write(const char* dir, const pxd_maps_meta* meta, const int32_t* excluded,
+                          const float* maps);
+/* Reads and checks meta.pxd (not the map payloads).  excluded (capacity cap)
+ * may be NULL. */
+pxd_status pxd_maps_info(const char* dir, pxd_maps_meta* meta, int32_t* excluded, int32_t cap);
+/* Reads the Coulomb map and the maps of types[0..n_types) (NULL: every saved
+ * type, in meta order) into maps, checking each payload's checksum.  With
+ * expect non-NULL the saved receptor (n_atoms, receptor_key, n_excluded and
+ * the excluded atoms) must equal expect's and excluded's. */
+pxd_status pxd_maps_read(const char* dir, const pxd_maps_meta* expect, const int32_t* excluded,
+                         const int32_t* types, int32_t n_types, float* maps, size_t n_floats);
+/* receptor_cache::save (grid.cpp:309-344): every map of g and its metadata,
+ * for the receptor g was built on.  Creates dir when it does not exist. */
+pxd_status pxd_grid_save(const pxd_grid* g, const char* dir);
+/* receptor_cache::restore (grid.cpp:346-391) into a device grid of r: the
+ * Coulomb map and the maps of types[0..n_types) (NULL: every saved type).
+ * The maps must have been saved for r (same key and movable atoms).  slope < 0
+ * takes the saved slope. */
+pxd_status pxd_grid_load(const pxd_receptor* r, const char* dir, const int32_t* types,
+                         int32_t n_types, double slope, pxd_grid** out);
 
 /* ------------------------------------------------------------------------ */
 /* Ligand batch (free_ligand, data.h:73-99) packed for uneven sizes          */

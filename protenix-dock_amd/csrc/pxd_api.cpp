@@ -608,6 +608,9 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   r->n_leaves = nl;
   r->n_mov = M;
   r->h_xyz = nullptr;
+  r->movable.assign(mov_atom.begin(), mov_atom.end());
+  std::sort(r->movable.begin(), r->movable.end());
+  (void)pxd_receptor_key(m, xyz, lv, &r->key);
   pxd_status st = build_leaf_tables(r);
   if (st != PXD_OK) {
     pxd_receptor_destroy(r);
@@ -685,6 +688,7 @@ static pxd_status grid_alloc(const pxd_receptor* r, const pxd_box* box, const in
   D.inv_spacing = (float)(1.0 / box->spacing);
   D.inv_spacing_d = 1.0 / box->spacing;
   D.slope = (float)slope;
+  g->slope = slope;
   D.P = P;
   D.maps = reinterpret_cast<float4*>(static_cast<char*>(g->mem) + kGridHeader);
   D.layout = 0;
@@ -839,6 +843,7 @@ pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types) {
 pxd_status pxd_grid_set_slope(pxd_grid* g, double slope) {
   if (!g) return fail(PXD_ERR_ARG, "grid slope: null grid");
   g->dev.slope = (float)slope;
+  g->slope = slope;
   return grid_sync_struct(g);
 }
 

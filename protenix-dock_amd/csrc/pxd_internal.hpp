@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "pxdock.h"
 
 namespace pxd {
@@ -191,6 +193,8 @@ struct pxd_receptor {
   float* h_xyz;               // initial coordinates
   int n_leaves;
   int n_mov;
+  uint64_t key;               // pxd_receptor_key of the inputs
+  std::vector<int32_t> movable;  // atoms of the leaves, ascending
 };
 
 struct pxd_grid {
@@ -201,6 +205,7 @@ struct pxd_grid {
   int32_t types[PXD_NUM_VDW_TYPES];
   void* mem;
   const pxd_receptor* receptor;
+  double slope;               // out-of-box wall slope as given (dev.slope is its FP32)
 };
 
 struct pxd_ligand_batch {

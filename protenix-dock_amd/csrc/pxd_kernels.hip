@@ -59,6 +59,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   __shared__ float4 s_nat[kNearCap]; // candidate x, y, z, R^2
   __shared__ int s_nidx[kNearCap];
   __shared__ int s_nn;
+  __shared__ int s_wcnt[kTile / kWave];  // candidates per wave of the current tile
   const int ny = A.np[1], nz = A.np[2];
   const float ix = A.init[0], iy = A.init[1], iz = A.init[2], spacing = A.spacing;
   long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -103,28 +104,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   for (int base = 0; base < A.n_atoms; base += kTile) {
     __syncthreads();
     int a = base + threadIdx.x;
+    bool cand = false;
+    float4 at = make_float4(0.f, 0.f, 0.f, 0.f);
+    float r2 = 0.f;
     if (a < A.n_atoms) {
-      float4 at = A.atoms[a];
+      at = A.atoms[a];
       float2 lj = A.alj[a];
       s_at[threadIdx.x] = at;
       s_lj[threadIdx.x] = lj;
       const float R = 0.5f * (lj.x + s_bb[6]);
-      const float r2 = R * R;
+      r2 = R * R;
       float dx = fmaxf(fmaxf(s_bb[0] - at.x, at.x - s_bb[3]), 0.f);
       float dy = fmaxf(fmaxf(s_bb[1] - at.y, at.y - s_bb[4]), 0.f);
       float dz = fmaxf(fmaxf(s_bb[2] - at.z, at.z - s_bb[5]), 0.f);
+      cand = dx * dx + dy * dy + dz * dz < 1.01f * r2 + 1e-3f;
+    }
+    // candidates take list slots in atom order (a prefix count, not an
+    // atomic counter), so the FP64 near sums below add in a fixed order and
+    // the maps are the same bits on every build
+    const uint64_t bal = __ballot(cand);
+    const int wave = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) s_wcnt[wave] = __popcll(bal);
+    __syncthreads();
+    int slot = s_nn + __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    for (int w = 0; w < wave; ++w) slot += s_wcnt[w];
+    if (a < A.n_atoms) {
       float keep = -1.f;
-      if (dx * dx + dy * dy + dz * dz < 1.01f * r2 + 1e-3f) {
-        int slot = atomicAdd(&s_nn, 1);
-        if (slot < kNearCap) {
-          s_nat[slot] = make_float4(at.x, at.y, at.z, r2);
-          s_nidx[slot] = a;
-          keep = r2;
-        }
+      if (cand && slot < kNearCap) {
+        s_nat[slot] = make_float4(at.x, at.y, at.z, r2);
+        s_nidx[slot] = a;
+        keep = r2;
       }
       s_r2[threadIdx.x] = keep;
     }
-    __syncthreads();
+    __syncthreads();  // every thread has read s_nn and the tile is staged
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int w = 0; w < kTile / kWave; ++w) tot += s_wcnt[w];
+      s_nn += tot;  // read again only after the next barrier
+    }
     int cnt = A.n_atoms - base < kTile ? A.n_atoms - base : kTile;
     for (int u = 0; u < cnt; ++u) {
       float4 at = s_at[u];
@@ -160,6 +179,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       }
     }
   }
+  __syncthreads();  // the last tile's count is in s_nn
   if (p >= A.P) return;
   const int nn = s_nn < kNearCap ? s_nn : kNearCap;
   const double qx = A.init64[0] + A.spacing64 * i, qy = A.init64[1] + A.spacing64 * j,

@@ -79,6 +79,22 @@ class PxdBox(C.Structure):
         return n[0] * n[1] * n[2]
 
 
+MAPS_VERSION = 1
+
+
+class PxdMapsMeta(C.Structure):
+    """pxd_maps_meta: metadata of a saved map directory."""
+    _fields_ = [
+        ("box", PxdBox), ("slope", C.c_double), ("n_atoms", C.c_int32),
+        ("n_excluded", C.c_int32), ("receptor_key", C.c_uint64), ("n_types", C.c_int32),
+        ("types", C.c_int32 * NUM_VDW_TYPES),
+    ]
+
+    @property
+    def type_list(self):
+        return [int(self.types[i]) for i in range(self.n_types)]
+
+
 class PxdScoreParams(C.Structure):
     _fields_ = [
         ("coul_coef", C.c_double), ("vdw_coef", C.c_double), ("ionic_coef", C.c_double),

@@ -76,6 +76,13 @@ def load():
         "pxd_mc_search_conformers": (i, [P, P, P, P, P, P, i, i32p, P, P, P, P, P, P, i, d, P,
                                          P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
+        "pxd_vdw_type_tag": (C.c_char_p, [i]),
+        "pxd_receptor_key": (i, [P, f64p, P, C.POINTER(C.c_uint64)]),
+        "pxd_maps_write": (i, [C.c_char_p, P, i32p, f32p]),
+        "pxd_maps_info": (i, [C.c_char_p, P, i32p, i]),
+        "pxd_maps_read": (i, [C.c_char_p, P, i32p, i32p, i, f32p, C.c_size_t]),
+        "pxd_grid_save": (i, [P, C.c_char_p]),
+        "pxd_grid_load": (i, [P, C.c_char_p, i32p, i, d, PP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -168,6 +175,32 @@ class Grid:
                                          C.byref(h)))
         self.h = h
 
+    @classmethod
+    def load(cls, receptor, path, types=None, slope=None):
+        """receptor_cache::restore (grid.cpp:346-391): maps saved by save() for
+        this receptor; types None loads every saved type."""
+        g = cls.__new__(cls)
+        g.receptor = receptor
+        h = C.c_void_p()
+        if types is None:
+            tp, n = None, 0
+        else:
+            g_types = np.ascontiguousarray(sorted(set(int(t) for t in types)), dtype=np.int32)
+            tp, n = g_types.ctypes.data_as(i32p), len(g_types)
+        check(load().pxd_grid_load(receptor.h, os.fsencode(path), tp, n,
+                                   -1.0 if slope is None else float(slope), C.byref(h)))
+        g.h = h
+        b, nt = ffi.PxdBox(), C.c_int32()
+        check(load().pxd_grid_info(h, _ref(b), C.byref(nt)))
+        g.box = b
+        g.types = (g_types if types is not None
+                   else np.ascontiguousarray(maps_info(path)[0].type_list, dtype=np.int32))
+        return g
+
+    def save(self, path):
+        """receptor_cache::save (grid.cpp:309-344) in the map-directory format."""
+        check(load().pxd_grid_save(self.h, os.fsencode(path)))
+
     def download(self):
         n = (len(self.types) + 1) * self.box.n_points() * 4
         out = np.empty(n, np.float32)
@@ -185,6 +218,70 @@ class Grid:
         if getattr(self, "h", None) and _lib is not None:
             _lib.pxd_grid_destroy(self.h)
             self.h = None
+
+
+def type_tags():
+    """The 99 GAFF2 type tags in index order (kVdwTypeTags, constant.h:131-231)."""
+    return [load().pxd_vdw_type_tag(t).decode() for t in range(ffi.NUM_VDW_TYPES)]
+
+
+def receptor_key(mol):
+    """pxd_receptor_key: fingerprint of the fixed receptor atoms the maps sum."""
+    c = mol.to_c()
+    lv = mol.leaves.to_c() if mol.leaves is not None else None
+    xyz = ffi.as_f64(mol.xyz)
+    k = C.c_uint64()
+    check(load().pxd_receptor_key(_ref(c), xyz.ctypes.data_as(f64p),
+                                  _ref(lv) if lv is not None else None, C.byref(k)))
+    return k.value
+
+
+def maps_write(path, box, types, maps, n_atoms, excluded, key, slope=1e6):
+    """Host-only write of a map directory; maps is [(len(types)+1), P, 4] FP32."""
+    m = ffi.PxdMapsMeta()
+    m.box = box
+    m.slope = slope
+    m.n_atoms = n_atoms
+    ex = ffi.as_i32(excluded)
+    m.n_excluded = len(ex)
+    m.receptor_key = key
+    m.n_types = len(types)
+    for i, t in enumerate(types):
+        m.types[i] = int(t)
+    a = np.ascontiguousarray(maps, dtype=np.float32)
+    assert a.size == (len(types) + 1) * box.n_points() * 4
+    check(load().pxd_maps_write(os.fsencode(path), _ref(m), ex.ctypes.data_as(i32p),
+                                a.ctypes.data_as(f32p)))
+
+
+def maps_info(path):
+    """(PxdMapsMeta, excluded atoms) of a saved map directory (host only)."""
+    m = ffi.PxdMapsMeta()
+    check(load().pxd_maps_info(os.fsencode(path), _ref(m), None, 0))
+    ex = np.empty(max(m.n_excluded, 1), np.int32)
+    check(load().pxd_maps_info(os.fsencode(path), _ref(m), ex.ctypes.data_as(i32p), len(ex)))
+    return m, ex[:m.n_excluded].copy()
+
+
+def maps_read(path, types=None, expect=None):
+    """Host-only read: FP32 [(n+1), P, 4], map 0 Coulomb, then the requested
+    types (None: every saved type).  expect = (n_atoms, excluded, key) checks
+    that the maps belong to that receptor."""
+    meta, _ = maps_info(path)
+    sel = meta.type_list if types is None else [int(t) for t in types]
+    P = meta.box.n_points()
+    out = np.empty((len(sel) + 1, P, 4), np.float32)
+    tp = ffi.as_i32(sel if sel else [0])  # never a null list: null asks for every type
+    e, ex = None, None
+    if expect is not None:
+        e = ffi.PxdMapsMeta()
+        e.n_atoms, ex, e.receptor_key = expect[0], ffi.as_i32(expect[1]), expect[2]
+        e.n_excluded = len(ex)
+    check(load().pxd_maps_read(os.fsencode(path), _ref(e) if e is not None else None,
+                               ex.ctypes.data_as(i32p) if ex is not None else None,
+                               tp.ctypes.data_as(i32p), len(sel),
+                               out.ctypes.data_as(f32p), out.size))
+    return out
 
 
 class LigandBatch:
