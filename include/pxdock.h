@@ -571,6 +571,49 @@ pxd_status pxd_pscore_batch(const pxd_receptor* r, const pxd_ligand_batch* b,
                             double* terms, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Affinity score (bscore: root_scorer on the affinity_ranking tree,         */
+/* scorer.cpp:53-115, with gbsa_total_energy, scorer.cpp:735-866)            */
+
+/* leaves: the pscore leaves with the affinity tree's parameters, the
+ * torsion-strain weight being what the reference's coefficient map holds
+ * (the rotatable-energy coefficient is stored under the torsion-strain key,
+ * scorer.cpp:102-109: when both are active it replaces the torsion-strain
+ * weight, and the rotatable-energy leaf itself is never weighted).
+ * gbsa_coef: gbsa_energy_delta_coef; the term is E_com - E_lig - E_pro. */
+typedef struct pxd_bscore_params {
+  pxd_score_params leaves;
+  double gbsa_coef;
+} pxd_bscore_params;
+
+#define PXD_NUM_BSCORE_TERMS (PXD_NUM_TERMS + 3) /* leaves, then GB E_pro, E_lig, E_com */
+
+/* fill_atom_properties (scorer.cpp:735-758), host only: Born radius and
+ * screening factor of each atom from its atomic number, an H bonded (bonds
+ * [n_bonds][2], the JSON bond_index) to an N taking 1.3 A. */
+pxd_status pxd_gbsa_atom_params(int32_t n_atoms, const int32_t* atomic_numbers,
+                                int32_t n_bonds, const int32_t* bonds, double* radius,
+                                double* screen);
+
+/* Affinity score of each pose: the leaves (as pxd_pscore_batch with
+ * p->leaves) plus gbsa_coef * (E_com - E_lig - E_pro), the generalized-Born
+ * + surface-area energies (gbsa_total_energy::calculate, scorer.cpp:798-866)
+ * of the receptor (with the pose's receptor torsions), the ligand, and the
+ * complex (ligand atoms first, then the receptor's), in FP64 from FP64
+ * parameters and the pose's coordinates.  rec_radius / rec_screen: host
+ * [n_rec_atoms]; lig_radius / lig_screen: host, every atom of the batch in
+ * batch order (pxd_gbsa_atom_params per ligand, concatenated).  score:
+ * device [n_poses] (FP64); terms (may be NULL): device
+ * [n_poses][PXD_NUM_BSCORE_TERMS].  Allocates per call and returns when the
+ * results are in the caller's buffers. */
+pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
+                            const double* rec_screen, const pxd_ligand_batch* b,
+                            const double* lig_radius, const double* lig_screen,
+                            const pxd_bscore_params* p, int32_t n_poses,
+                            const int32_t* pose_lig, const int32_t* pose_atom_off,
+                            const float* xyz, const float* tors, double* score,
+                            double* terms, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Autotuning of launch parameters (no counterpart in the reference, whose    */
 /* CPU path has no launch parameters).                                       */
 

@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "pxdock_gbsa_table.h"
 #include "pxdock_vdw_table.h"
 
 /* system.h:83 */
@@ -1458,6 +1459,125 @@ double or_pscore(const pxd_score_params* sp, const pxd_mol* rec, const double* r
   double s = 0.0;
   for (int k = 0; k < PXD_NUM_TERMS; ++k)
     if (active(coef[k])) s += coef[k] * t[k];
+  if (terms) memcpy(terms, t, sizeof(t));
+  return s;
+}
+
+/* ======================================================================== */
+/* Affinity score: generalized Born (OBC) + surface area (scorer.cpp:735-866) */
+
+void or_gbsa_atom_params(int32_t n, const int32_t* an, int32_t n_bonds, const int32_t* bonds,
+                         double* radius, double* screen) {
+  static const double radii[] = PXD_GB_RADII;
+  static const double screens[] = PXD_GB_SCREENS;
+  for (int i = 0; i < n; ++i) radius[i] = -1.0;
+  for (int b = 0; b < n_bonds; ++b) {  /* scorer.cpp:742-749 */
+    int i = bonds[2 * b], j = bonds[2 * b + 1];
+    if (an[i] == 1 && an[j] == 7)
+      radius[i] = PXD_GB_RADIUS_HN;
+    else if (an[i] == 7 && an[j] == 1)
+      radius[j] = PXD_GB_RADIUS_HN;
+  }
+  for (int i = 0; i < n; ++i) {  /* scorer.cpp:751-757 */
+    if (radius[i] < 0.0) radius[i] = an[i] <= PXD_GB_NUM_RADII ? radii[an[i]] : radii[0];
+    screen[i] = an[i] <= PXD_GB_NUM_SCREENS ? screens[an[i]] : screens[0];
+  }
+}
+
+double or_gbsa_energy(int32_t n, const double* x, const double* q, const double* radius,
+                      const double* screen) {
+  double* orad = malloc(sizeof(double) * (n > 0 ? n : 1));
+  double* sum_i = calloc(n > 0 ? n : 1, sizeof(double));
+  double* born = malloc(sizeof(double) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) orad[i] = radius[i] - PXD_GB_OFFSET;
+  /* descreening sums (scorer.cpp:807-830): source i, target j */
+  for (int i = 0; i < n; ++i) {
+    double sr = orad[i] * screen[i], sr2 = sr * sr;
+    for (int j = 0; j < n; ++j) {
+      if (i == j) continue;
+      double r12 = dist3(x + 3 * i, x + 3 * j);
+      double U = r12 + sr;
+      if (U < orad[j]) continue;
+      double r12_inv = 1.0 / r12;
+      double D = fabs(r12 - sr);
+      double L = orad[j] > D ? orad[j] : D;
+      double U_inv = 1.0 / U, L_inv = 1.0 / L;
+      D = 0.25 * (r12 - sr2 * r12_inv) * (U_inv * U_inv - L_inv * L_inv);
+      D += L_inv - U_inv + 0.5 * log(L * U_inv + EPS) * r12_inv;
+      D *= 0.5;
+      sum_i[j] += D;
+    }
+  }
+  /* Born radii (scorer.cpp:832-839) */
+  for (int i = 0; i < n; ++i) {
+    double psi = sum_i[i] * orad[i], psi2 = psi * psi;
+    double th = tanh(PXD_GB_ALPHA * psi - PXD_GB_BETA * psi2 + PXD_GB_GAMMA * psi * psi2);
+    born[i] = 1.0 / (1.0 / orad[i] - th / radius[i]);
+  }
+  /* generalized-Born energy over all ordered pairs, self terms included
+   * (scorer.cpp:841-858) */
+  double egb = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      double f;
+      if (i == j) {
+        f = born[i];
+      } else {
+        double dx = x[3 * i] - x[3 * j], dy = x[3 * i + 1] - x[3 * j + 1],
+               dz = x[3 * i + 2] - x[3 * j + 2];
+        double r2 = dx * dx + dy * dy + dz * dz;  /* get_distance_square_3d */
+        double bp = born[i] * born[j];
+        f = sqrt(r2 + bp * exp(-0.25 * r2 / bp) + EPS);
+      }
+      double L = 0.5 * or_coulomb_factor *
+                 (1.0 / PXD_GB_SOLUTE_DIELECTRIC - exp(-PXD_GB_KAPPA * f) / PXD_GB_SOLVENT_DIELECTRIC);
+      egb -= q[i] * q[j] / f * L;
+    }
+  /* surface term (scorer.cpp:859-864) */
+  double esa = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double a = radius[i] + PXD_GB_PROBE_RADIUS;
+    esa += a * a * pow(radius[i] / born[i], 6.0);
+  }
+  free(orad);
+  free(sum_i);
+  free(born);
+  return egb + esa * PXD_GB_SURFACE_ENERGY;
+}
+
+double or_bscore(const pxd_bscore_params* bp, const pxd_mol* rec, const double* rx,
+                 const double* rec_radius, const double* rec_screen, const pxd_mol* lig,
+                 const double* lx, const double* lig_radius, const double* lig_screen,
+                 double* terms) {
+  double t[PXD_NUM_BSCORE_TERMS] = {0};
+  double s = or_pscore(&bp->leaves, rec, rx, lig, lx, t);
+  if (active(bp->gbsa_coef)) {
+    const int nr = rec->n_atoms, nl = lig->n_atoms, nc = nr + nl;
+    double* cx = malloc(sizeof(double) * 3 * nc);
+    double* cq = malloc(sizeof(double) * nc);
+    double* crad = malloc(sizeof(double) * nc);
+    double* cscr = malloc(sizeof(double) * nc);
+    /* the complex: ligand atoms first, then the receptor's (scorer.cpp:777-784) */
+    memcpy(cx, lx, sizeof(double) * 3 * nl);
+    memcpy(cx + 3 * nl, rx, sizeof(double) * 3 * nr);
+    memcpy(cq, lig->charge, sizeof(double) * nl);
+    memcpy(cq + nl, rec->charge, sizeof(double) * nr);
+    memcpy(crad, lig_radius, sizeof(double) * nl);
+    memcpy(crad + nl, rec_radius, sizeof(double) * nr);
+    memcpy(cscr, lig_screen, sizeof(double) * nl);
+    memcpy(cscr + nl, rec_screen, sizeof(double) * nr);
+    double ep = or_gbsa_energy(nr, rx, rec->charge, rec_radius, rec_screen);
+    double el = or_gbsa_energy(nl, lx, lig->charge, lig_radius, lig_screen);
+    double ec = or_gbsa_energy(nc, cx, cq, crad, cscr);
+    free(cx);
+    free(cq);
+    free(crad);
+    free(cscr);
+    t[PXD_NUM_TERMS] = ep;
+    t[PXD_NUM_TERMS + 1] = el;
+    t[PXD_NUM_TERMS + 2] = ec;
+    s += bp->gbsa_coef * (ec - el - ep);
+  }
   if (terms) memcpy(terms, t, sizeof(t));
   return s;
 }

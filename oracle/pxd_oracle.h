@@ -146,6 +146,22 @@ double or_pscore(const pxd_score_params* sp, const pxd_mol* rec, const double* r
 double or_pscore_vdw(const pxd_score_params* sp, const pxd_mol* rec, const double* rec_xyz,
                      const pxd_mol* lig, const double* lig_xyz);
 
+/* ---- affinity score: GBSA (scorer.cpp:735-866) and bscore ---- */
+/* fill_atom_properties (scorer.cpp:735-758) */
+void or_gbsa_atom_params(int32_t n, const int32_t* atomic_numbers, int32_t n_bonds,
+                         const int32_t* bonds, double* radius, double* screen);
+/* gbsa_total_energy::calculate (scorer.cpp:798-866) of one system */
+double or_gbsa_energy(int32_t n, const double* xyz, const double* charge, const double* radius,
+                      const double* screen);
+/* root_scorer on the affinity tree: the leaves of or_pscore with bp->leaves
+ * plus gbsa_coef * (E_com - E_lig - E_pro) (gbsa_total_energy::summary,
+ * scorer.cpp:760-796); terms[PXD_NUM_BSCORE_TERMS] = leaves, E_pro, E_lig,
+ * E_com (the GB entries 0 when the term is inactive). */
+double or_bscore(const pxd_bscore_params* bp, const pxd_mol* rec, const double* rec_xyz,
+                 const double* rec_radius, const double* rec_screen, const pxd_mol* lig,
+                 const double* lig_xyz, const double* lig_radius, const double* lig_screen,
+                 double* terms);
+
 /* ---- counter-based RNG shared with the GPU (Philox4x32-10) ---- */
 void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -83,6 +83,9 @@ def lib():
             "or_dof_energy": (d, [P, f64p, f64p, f64p, f64p]),
             "or_mc_mutate": (i, [P, P, i, i, C.POINTER(C.c_uint32), i, f64p]),
             "or_mc_draws": (None, [i, C.c_uint64, i, i, i, f64p]),
+            "or_gbsa_atom_params": (None, [i, i32p, i, i32p, f64p, f64p]),
+            "or_gbsa_energy": (d, [i, f64p, f64p, f64p, f64p]),
+            "or_bscore": (d, [P, P, f64p, f64p, f64p, P, f64p, f64p, f64p, f64p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -119,6 +122,9 @@ def ref():
         L.ref_score_params_compare.argtypes = [C.c_char_p, P, C.c_char_p, C.c_size_t]
         L.ref_pscore_json.restype = d
         L.ref_pscore_json.argtypes = [C.c_char_p, P, f64p, P, f64p]
+        L.ref_bscore_json.restype = d
+        L.ref_bscore_json.argtypes = [C.c_char_p, P, f64p, i32p, i32p, C.c_int, P, f64p, i32p,
+                                      i32p, C.c_int, f64p]
         _ref = L
     return _ref
 
@@ -274,6 +280,47 @@ def pscore(sp, receptor, rec_xyz, ligand, lig_xyz, use_ref=False):
     L = ref() if use_ref else lib()
     fn = L.ref_pscore if use_ref else L.or_pscore
     s = fn(byref(sp), byref(rc), dp(f64(rec_xyz)), byref(lc), dp(f64(lig_xyz)), dp(terms))
+    return s, terms
+
+
+def gbsa_atom_params(mol):
+    """(radius, screen) of every atom (fill_atom_properties, scorer.cpp:735-758)."""
+    an = np.ascontiguousarray(mol.atomic_numbers, np.int32)
+    bonds = np.ascontiguousarray(np.asarray(mol.all_bonds, np.int32).reshape(-1, 2))
+    r, sc = np.zeros(len(an)), np.zeros(len(an))
+    lib().or_gbsa_atom_params(len(an), ip(an), len(bonds), ip(bonds if len(bonds) else
+                                                             np.zeros((1, 2), np.int32)),
+                              dp(r), dp(sc))
+    return r, sc
+
+
+def gbsa_energy(xyz, charge, radius, screen):
+    x = f64(xyz).reshape(-1, 3)
+    return lib().or_gbsa_energy(len(x), dp(x), dp(f64(charge)), dp(f64(radius)), dp(f64(screen)))
+
+
+def bscore(bp, receptor, rec_xyz, ligand, lig_xyz, rec_gb=None, lig_gb=None, use_ref=None):
+    """Affinity score and its terms (leaves, E_pro, E_lig, E_com).  use_ref: the
+    affinity tree's JSON text, to score with the reference's compiled code."""
+    rc, lc = receptor.to_c(), ligand.to_c()
+    rx, lx = f64(rec_xyz).reshape(-1, 3), f64(lig_xyz).reshape(-1, 3)
+    if use_ref is not None:
+        gb = np.zeros(3)
+        rb = np.ascontiguousarray(np.asarray(receptor.all_bonds, np.int32).reshape(-1, 2))
+        lb = np.ascontiguousarray(np.asarray(ligand.all_bonds, np.int32).reshape(-1, 2))
+        s = ref().ref_bscore_json(use_ref.encode(), byref(rc), dp(rx),
+                                  ip(np.ascontiguousarray(receptor.atomic_numbers, np.int32)),
+                                  ip(rb if len(rb) else np.zeros((1, 2), np.int32)), len(rb),
+                                  byref(lc), dp(lx),
+                                  ip(np.ascontiguousarray(ligand.atomic_numbers, np.int32)),
+                                  ip(lb if len(lb) else np.zeros((1, 2), np.int32)), len(lb),
+                                  dp(gb))
+        return s, gb
+    rr, rs = rec_gb if rec_gb is not None else gbsa_atom_params(receptor)
+    lr, ls = lig_gb if lig_gb is not None else gbsa_atom_params(ligand)
+    terms = np.zeros(ffi.NUM_BSCORE_TERMS)
+    s = lib().or_bscore(byref(bp), byref(rc), dp(rx), dp(f64(rr)), dp(f64(rs)), byref(lc),
+                        dp(lx), dp(f64(lr)), dp(f64(ls)), dp(terms))
     return s, terms
 
 

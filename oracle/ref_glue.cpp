@@ -222,15 +222,26 @@ ScoringFunctionParams sfp_from_json(const pxdock::json& n) {
   PXD_RD0(n, f, hbond_energy_coef);
   PXD_RD0(n, f, hydrophobic_energy_coef);
   PXD_RD0(n, f, torsion_strain_penalty_coef);
+  PXD_RD0(n, f, rotatable_energy_coef);
+  PXD_RD0(n, f, gbsa_energy_delta_coef);
   if (active_coef(f.interaction_vdw_energy_coef)) {
     const auto& v = n["interaction_vdw_energy"];
     PXD_RD(v, f.interaction_vdw_energy, scale);
     PXD_RD(v, f.interaction_vdw_energy, cutoff);
     const auto& ty = v["types"];
-    f.interaction_vdw_energy.types.resize(ty.size());
-    for (size_t t = 0; t < ty.size(); ++t) {
-      PXD_RD(ty[t], f.interaction_vdw_energy.types[t], sigma_scale);
-      PXD_RD(ty[t], f.interaction_vdw_energy.types[t], epsilon_scale);
+    // the leaf's constructor reads 99 entries (scorer.cpp:210-217); a shorter
+    // list (the default affinity tree has 38) is padded with scale 1 here so
+    // the reference code reads defined values
+    const size_t nt = ty.size() < 99 ? 99 : ty.size();
+    f.interaction_vdw_energy.types.resize(nt);
+    for (size_t t = 0; t < nt; ++t) {
+      if (t < ty.size()) {
+        PXD_RD(ty[t], f.interaction_vdw_energy.types[t], sigma_scale);
+        PXD_RD(ty[t], f.interaction_vdw_energy.types[t], epsilon_scale);
+      } else {
+        f.interaction_vdw_energy.types[t].sigma_scale = 1.0;
+        f.interaction_vdw_energy.types[t].epsilon_scale = 1.0;
+      }
     }
   }
   if (active_coef(f.ionic_energy_coef)) {
@@ -415,6 +426,32 @@ double ref_pscore_json(const char* json_text, const pxd_mol* rec, const double* 
   root_scorer root("pscore", sfp_from_json(pxdock::json::parse(json_text)));
   instant_cache memo = root.precalculate(rx, rf, lx, lf);
   return root.combine(root.summary(rx, rf, lx, lf, &memo));
+}
+
+// root_scorer("bscore") on an affinity_ranking JSON tree (scorer.cpp:53-115,
+// gbsa_total_energy::summary scorer.cpp:760-796) with the molecules' atomic
+// numbers and bond lists (the GBSA radii, scorer.cpp:735-758); gb receives
+// Gbsa_Pro_Energy, Gbsa_Lig_Energy, Gbsa_Com_Energy (0 when inactive).
+double ref_bscore_json(const char* json_text, const pxd_mol* rec, const double* rec_xyz,
+                       const int32_t* rec_an, const int32_t* rec_bonds, int n_rec_bonds,
+                       const pxd_mol* lig, const double* lig_xyz, const int32_t* lig_an,
+                       const int32_t* lig_bonds, int n_lig_bonds, double* gb) {
+  force_field_params rf = to_ffp(rec), lf = to_ffp(lig);
+  rf.atomic_numbers.assign(rec_an, rec_an + rec->n_atoms);
+  lf.atomic_numbers.assign(lig_an, lig_an + lig->n_atoms);
+  fill_ids(rf.all_bonds, rec_bonds, n_rec_bonds, 2);
+  fill_ids(lf.all_bonds, lig_bonds, n_lig_bonds, 2);
+  lf.rotatable_bonds.resize(lig->n_rotatable);
+  molecule_pose rx = to_pose(rec_xyz, rec->n_atoms), lx = to_pose(lig_xyz, lig->n_atoms);
+  root_scorer root("bscore", sfp_from_json(pxdock::json::parse(json_text)));
+  instant_cache memo = root.precalculate(rx, rf, lx, lf);
+  auto scores = root.summary(rx, rf, lx, lf, &memo);
+  static const char* names[3] = {"Gbsa_Pro_Energy", "Gbsa_Lig_Energy", "Gbsa_Com_Energy"};
+  for (int k = 0; k < 3; ++k) {
+    auto it = scores.find(names[k]);
+    gb[k] = it == scores.end() ? 0.0 : it->second;
+  }
+  return root.combine(scores);
 }
 
 // kVdwTypeDatabase (constant.h:29-130)

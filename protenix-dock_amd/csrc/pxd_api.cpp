@@ -610,6 +610,15 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   r->h_xyz = nullptr;
   r->movable.assign(mov_atom.begin(), mov_atom.end());
   std::sort(r->movable.begin(), r->movable.end());
+  r->h_xyz64.assign(xyz, xyz + 3 * (size_t)n);
+  r->h_q64.assign(m->charge, m->charge + n);
+  r->h_mov_atom = mov_atom;
+  r->h_mov_leaf = mov_leaf;
+  for (int k = 0; k < nl; ++k)
+    for (int d = 0; d < 3; ++d) {
+      r->h_leaf_o.push_back(lv->origin[3 * k + d]);
+      r->h_leaf_a.push_back(lv->axis[3 * k + d]);
+    }
   (void)pxd_receptor_key(m, xyz, lv, &r->key);
   pxd_status st = build_leaf_tables(r);
   if (st != PXD_OK) {
@@ -863,6 +872,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   std::vector<LigInfo> info(nlig);
   std::vector<int> type;
   std::vector<float> q;
+  std::vector<double> q64;
   std::vector<float2> lj;
   std::vector<uint32_t> bij, aijk, dijkl, dper, pairs, tsijkl;
   std::vector<float2> bkr, akt;
@@ -903,6 +913,7 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
       used[t] = 1;
       type.push_back(t);
       q.push_back((float)m.charge[i]);
+      q64.push_back(m.charge[i]);
       lj.push_back(make_float2((float)kGaffSigma[t], (float)std::sqrt(kGaffEpsilon[t])));
     }
     I.bond_off = (int)bij.size();
@@ -1173,6 +1184,9 @@ pxd_status pxd_ligand_batch_create(const pxd_mol* ligs, const pxd_frag_tree* tre
   memcpy(b->h_ntors, ntors.data(), sizeof(int) * nlig);
   memcpy(b->type_used, used, sizeof(used));
   b->has_trees = trees ? 1 : 0;
+  b->h_q64 = std::move(q64);
+  b->h_atom_off.resize(nlig + 1, 0);
+  for (int l = 0; l < nlig; ++l) b->h_atom_off[l + 1] = b->h_atom_off[l] + natoms[l];
   *out = b;
   return PXD_OK;
 }

@@ -195,6 +195,10 @@ struct pxd_receptor {
   int n_mov;
   uint64_t key;               // pxd_receptor_key of the inputs
   std::vector<int32_t> movable;  // atoms of the leaves, ascending
+  // FP64 inputs for the host-staged FP64 terms (the affinity score's GBSA)
+  std::vector<double> h_xyz64, h_q64;      // [n][3], [n]
+  std::vector<double> h_leaf_o, h_leaf_a;  // [n_leaves][3] origin, unit axis
+  std::vector<int> h_mov_atom, h_mov_leaf; // movable slots in leaf order
 };
 
 struct pxd_grid {
@@ -219,4 +223,6 @@ struct pxd_ligand_batch {
   int max_frags;              // most fragments of a ligand
   int has_trees;              // fragment trees were given (pose parameterisation)
   uint8_t type_used[PXD_NUM_VDW_TYPES];
+  std::vector<double> h_q64;  // FP64 charges of every atom, batch order
+  std::vector<int> h_atom_off; // first atom of each ligand in h_q64
 };

@@ -116,6 +116,14 @@ class PxdScoreParams(C.Structure):
     ]
 
 
+NUM_BSCORE_TERMS = NUM_TERMS + 3  # leaves, then GB E_pro, E_lig, E_com
+
+
+class PxdBscoreParams(C.Structure):
+    """pxd_bscore_params: the affinity tree's leaves and the GBSA weight."""
+    _fields_ = [("leaves", PxdScoreParams), ("gbsa_coef", C.c_double)]
+
+
 class PxdLbfgsParams(C.Structure):
     _fields_ = [
         ("history", C.c_int32), ("max_iter", C.c_int32), ("max_eval", C.c_int32),

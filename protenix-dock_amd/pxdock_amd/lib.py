@@ -77,6 +77,8 @@ def load():
                                          P, P]),
         "pxd_apply_poses": (i, [P, i, P, P, P, P, P, P, P, P, P]),
         "pxd_vdw_type_tag": (C.c_char_p, [i]),
+        "pxd_gbsa_atom_params": (i, [i, i32p, i, i32p, f64p, f64p]),
+        "pxd_bscore_batch": (i, [P, f64p, f64p, P, f64p, f64p, P, i, P, P, P, P, P, P, P]),
         "pxd_receptor_key": (i, [P, f64p, P, C.POINTER(C.c_uint64)]),
         "pxd_maps_write": (i, [C.c_char_p, P, i32p, f32p]),
         "pxd_maps_info": (i, [C.c_char_p, P, i32p, i]),
@@ -410,6 +412,44 @@ def pscore(receptor, poses, params, with_terms=True):
     t = torch.empty((poses.n_poses, ffi.NUM_TERMS), dtype=torch.float64, device=dev) \
         if with_terms else None
     check(load().pxd_pscore_batch(receptor.h, poses.batch.h, _ref(params), poses.n_poses,
+                                  _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
+                                  _dev(poses.tors), _dev(s), _dev(t), _stream()))
+    return s, t
+
+
+def gbsa_atom_params(mol):
+    """pxd_gbsa_atom_params (host): Born radius and screening factor per atom."""
+    an = np.ascontiguousarray(mol.atomic_numbers, dtype=np.int32)
+    if len(an) != mol.n_atoms:
+        raise ValueError("GBSA needs the atomic number of every atom")
+    bonds = np.ascontiguousarray(np.asarray(mol.all_bonds, np.int32).reshape(-1, 2))
+    r, sc = np.zeros(len(an)), np.zeros(len(an))
+    check(load().pxd_gbsa_atom_params(len(an), an.ctypes.data_as(i32p), len(bonds),
+                                      bonds.ctypes.data_as(i32p) if len(bonds) else None,
+                                      r.ctypes.data_as(f64p), sc.ctypes.data_as(f64p)))
+    return r, sc
+
+
+def bscore(receptor, poses, params, with_terms=True, rec_gb=None, lig_gb=None):
+    """pxd_bscore_batch: affinity score per pose and its terms [n, NUM_BSCORE_TERMS]
+    (leaves, E_pro, E_lig, E_com).  rec_gb / lig_gb: (radius, screen) of the
+    receptor and of the batch's atoms in batch order (default: from the
+    molecules' atomic numbers and bonds)."""
+    import torch
+    if rec_gb is None:
+        rec_gb = gbsa_atom_params(receptor.mol)
+    if lig_gb is None:
+        parts = [gbsa_atom_params(m) for m in poses.batch.ligands]
+        lig_gb = (np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]))
+    rr, rs = (np.ascontiguousarray(a, np.float64) for a in rec_gb)
+    lr, ls = (np.ascontiguousarray(a, np.float64) for a in lig_gb)
+    dev = poses.xyz.device
+    s = torch.empty(poses.n_poses, dtype=torch.float64, device=dev)
+    t = torch.empty((poses.n_poses, ffi.NUM_BSCORE_TERMS), dtype=torch.float64, device=dev) \
+        if with_terms else None
+    check(load().pxd_bscore_batch(receptor.h, rr.ctypes.data_as(f64p), rs.ctypes.data_as(f64p),
+                                  poses.batch.h, lr.ctypes.data_as(f64p),
+                                  ls.ctypes.data_as(f64p), _ref(params), poses.n_poses,
                                   _dev(poses.d_lig), _dev(poses.d_off), _dev(poses.xyz),
                                   _dev(poses.tors), _dev(s), _dev(t), _stream()))
     return s, t

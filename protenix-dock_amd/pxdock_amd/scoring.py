@@ -23,8 +23,14 @@ def active(c):
     return c > 1e-5 or c < -1e-5
 
 
-def params_from_dict(node):
-    """Fill a PxdScoreParams from one scoring tree (e.g. config['pose_selection'])."""
+def params_from_dict(node, pad_vdw_types=False):
+    """Fill a PxdScoreParams from one scoring tree (e.g. config['pose_selection']).
+
+    The reference reads 99 vdW type scales whatever the list holds
+    (scorer.cpp:210-217); a shorter list is an error here unless
+    pad_vdw_types, which gives the missing types scale 1 (the affinity tree of
+    the default configuration lists 38 types, all of scale 1, and the
+    reference's read past them is undefined)."""
     p = ffi.PxdScoreParams()
     for t in range(ffi.NUM_VDW_TYPES):
         p.vdw_sigma_scale[t] = 1.0
@@ -44,10 +50,10 @@ def params_from_dict(node):
         p.vdw_scale = float(v["scale"])
         p.vdw_cutoff = float(v["cutoff"])
         types = v["types"]
-        if len(types) < ffi.NUM_VDW_TYPES:
+        if len(types) < ffi.NUM_VDW_TYPES and not pad_vdw_types:
             # the reference reads 99 entries regardless (scorer.cpp:213-217)
             raise ValueError("interaction_vdw_energy.types needs 99 entries")
-        for t in range(ffi.NUM_VDW_TYPES):
+        for t in range(min(len(types), ffi.NUM_VDW_TYPES)):
             p.vdw_sigma_scale[t] = float(types[t]["sigma_scale"])
             p.vdw_epsilon_scale[t] = float(types[t]["epsilon_scale"])
     if active(p.ionic_coef):
@@ -88,6 +94,30 @@ def params_from_dict(node):
         p.hydro_dist_max = float(v["dist_max"])
         p.hydro_scale_factor = float(v["scale_factor"])
     return p
+
+
+def bscore_params_from_dict(node):
+    """pxd_bscore_params from an affinity_ranking tree, with the weights the
+    reference's root_scorer ends up using (scorer.cpp:53-115): the
+    rotatable-energy coefficient is stored under the torsion-strain key, so
+    when both are active it replaces the torsion-strain weight, and the
+    rotatable-energy leaf (named 'rotatable_energy') has no weight at all;
+    molecular_weight_reward has no scorer.  The GBSA delta is weighted by
+    gbsa_energy_delta_coef."""
+    b = ffi.PxdBscoreParams()
+    b.leaves = params_from_dict(node, pad_vdw_types=True)
+    rot = float(node.get("rotatable_energy_coef", 0.0))
+    if active(b.leaves.tstrain_coef) and active(rot):
+        b.leaves.tstrain_coef = rot
+    b.gbsa_coef = float(node.get("gbsa_energy_delta_coef", 0.0))
+    return b
+
+
+def load_affinity_ranking(path=None):
+    """scoring_function_factory (factory.cpp:213-238), affinity_ranking tree."""
+    with open(path or DEFAULT_CONFIG) as fh:
+        cfg = json.load(fh)
+    return bscore_params_from_dict(cfg["affinity_ranking"])
 
 
 def load_pose_selection(path=None):
