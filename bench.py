@@ -299,6 +299,64 @@ def large_leg(args, workers):
     return d
 
 
+def bscore_leg(args, rec, R, B, ligs, pose_lig, xyz_list, threads):
+    """Affinity score (bscore: the affinity tree's pscore leaves + the FP64
+    GBSA delta over receptor, ligand and complex) of an evenly strided sample
+    of the main workload's poses, as bdscr / the result writers compute it;
+    the FP64 oracle on a bounded sample of the same poses on every host core."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+
+    import pyoracle as O
+    from pxdock_amd import lib, scoring
+    bp = scoring.load_affinity_ranking()
+    n = min(args.bscore_poses, len(pose_lig))
+    sel = np.linspace(0, len(pose_lig) - 1, n).astype(int)
+    P = lib.PoseSet(B, pose_lig[sel], [xyz_list[i] for i in sel], rec.leaves.n_leaves)
+    rec_gb = lib.gbsa_atom_params(rec)
+    parts = [lib.gbsa_atom_params(m) for m in ligs]
+    lig_gb = (np.concatenate([q[0] for q in parts]), np.concatenate([q[1] for q in parts]))
+    w = lib.PoseSet(B, pose_lig[sel[:16]], [xyz_list[i] for i in sel[:16]], rec.leaves.n_leaves)
+    lib.bscore(R, w, bp, rec_gb=rec_gb, lig_gb=lig_gb)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    s, _ = lib.bscore(R, P, bp, rec_gb=rec_gb, lig_gb=lig_gb)
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    na = np.array([ligs[pose_lig[i]].n_atoms for i in sel], np.float64)
+    N = float(rec.n_atoms)
+    # ordered atom pairs of the descreening and GB passes over the three systems
+    pairs = float((2.0 * ((na + N) ** 2 + N * N + na * na)).sum())
+    gpu = {"poses": int(n), "seconds": dt, "poses_per_s": n / dt,
+           "gb_pair_terms_per_s": pairs / dt, "finite": bool(np.isfinite(s.cpu().numpy()).all())}
+    # the oracle: one pose per thread
+    rr, rs = rec_gb
+    lo = np.concatenate([[0], np.cumsum([m.n_atoms for m in ligs])])
+
+    def one(i):
+        l = int(pose_lig[i])
+        return O.bscore(bp, rec, rec.xyz, ligs[l], np.asarray(xyz_list[i], np.float64),
+                        rec_gb=(rr, rs), lig_gb=(lig_gb[0][lo[l]:lo[l + 1]],
+                                                 lig_gb[1][lo[l]:lo[l + 1]]))[0]
+    t1 = time.time()
+    one(int(sel[0]))
+    per = time.time() - t1
+    nc = int(max(threads, min(n, args.cpu_seconds / max(per, 1e-6) * threads)))
+    cs = sel[np.linspace(0, n - 1, nc).astype(int)]
+    t2 = time.time()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(one, [int(i) for i in cs]))
+    dtc = time.time() - t2
+    d = {"workload": f"{n} poses of the batched score+grad workload (receptor {rec.n_atoms} "
+                     f"atoms), default affinity tree",
+         "gpu": gpu,
+         "cpu": {"poses": nc, "seconds": dtc, "cores": threads, "kind": "port",
+                 "poses_per_s": nc / dtc},
+         "speedup_vs_cpu": gpu["poses_per_s"] / (nc / dtc)}
+    log(f"bscore: {d}")
+    return d
+
+
 def _make_train_ligand(i):
     from pxdock_amd import synth
     rng = np.random.default_rng(40000 + i + 7919)
@@ -405,6 +463,8 @@ def main():
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-bscore", action="store_true")
+    ap.add_argument("--bscore-poses", type=int, default=1024)
     ap.add_argument("--large-ligands", type=int, default=4096)
     ap.add_argument("--train-poses", type=int, default=16384)
     ap.add_argument("--search-ligands", type=int, default=256)
@@ -665,6 +725,8 @@ def main():
     # full-search workload: 256 ligands x 64 chains; steps cut from 2,000)
     if not args.no_search:
         out["search"] = search_leg(args, rec, box, R, G, workers)
+    if not args.no_bscore:
+        out["bscore"] = bscore_leg(args, rec, R, B, ligs, pose_lig, xyz_list, host_cores())
     if not args.no_train:
         out["train"] = train_leg(args, rec, R, workers)
     if not args.no_large:
