@@ -276,10 +276,13 @@ pxd_status pxd_grid_upload(const pxd_receptor* r, const pxd_box* box, const int3
 pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats);
 pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types);
 /* Memory layout of the maps read by the lookups (receptor_cache::get,
- * grid.cpp:197-303, gives identical results in both):
- *   0  point-major, as built and as uploaded/downloaded (default);
+ * grid.cpp:197-303, gives identical results in all three):
+ *   0  point-major, each point one (value, dE/dx, dE/dy, dE/dz) record, as
+ *      built and as uploaded/downloaded (default);
  *   1  every cell's 8 corner points stored together, so one lookup reads one
- *      128-byte line per map; costs 8x the map memory, kept beside layout 0.
+ *      128-byte line per map; costs 8x the map memory, kept beside layout 0;
+ *   2  point-major component planes (value, dE/dx, dE/dy, dE/dz each a
+ *      separate array); costs 1x the map memory, kept beside layout 0.
  * Waits for the stream; the choice applies to later launches. */
 pxd_status pxd_grid_set_layout(pxd_grid* g, int32_t layout, void* stream);
 pxd_status pxd_grid_set_slope(pxd_grid* g, double slope);
@@ -621,20 +624,24 @@ typedef struct pxd_tuning {
   int32_t grid_types_per_pass; /* type maps per pass of the grid build: 4, 8, 16 */
   int32_t map_layout;          /* lookup layout applied to new grids (pxd_grid_set_layout) */
   int32_t lbfgs_slots_per_cu;  /* resident L-BFGS work slots per compute unit */
+  int32_t score_poses_per_block; /* poses (one wave each) per workgroup of the batched
+                                    score+grad: 1, 2, 4 */
   double grid_ms[3];           /* measured build times for 4, 8, 16 types per pass */
-  double score_ms[2];          /* measured score+grad times for layouts 0, 1 */
+  double score_ms[3];          /* measured score+grad times for layouts 0, 1, 2 */
   double lbfgs_ms[4];          /* measured L-BFGS times for 2, 4, 8, 16 slots per CU */
+  double score_ppb_ms[3];      /* measured score+grad times for 1, 2, 4 poses per block */
 } pxd_tuning;
 
-/* The process-wide choice in force (defaults: 8, 0, 8; timings zero). */
+/* The process-wide choice in force (defaults: 8, 0, 8, 1; timings zero). */
 void pxd_get_tuning(pxd_tuning* out);
-/* Sets the three choices (timings ignored). */
+/* Sets the four choices (timings ignored). */
 pxd_status pxd_set_tuning(const pxd_tuning* t);
 /* Times the variants on the caller's receptor, box, types, ligands and poses
  * (device arrays, left unchanged) and keeps the fastest of each: grid build
- * types per pass, map layout for score+grad, L-BFGS slots per CU (40
- * iterations from the given poses).  Results of every variant are identical;
- * only their speed differs. */
+ * types per pass, map layout for score+grad, poses per workgroup of
+ * score+grad (on the chosen layout), L-BFGS slots per CU (40 iterations
+ * from the given poses).  Results of every variant are identical; only
+ * their speed differs. */
 pxd_status pxd_autotune(const pxd_receptor* r, const pxd_box* box, const int32_t* types,
                         int32_t n_types, const pxd_ligand_batch* b, int32_t n_poses,
                         const int32_t* pose_lig, const int32_t* pose_atom_off, const float* xyz,

@@ -200,6 +200,7 @@ struct Tuning {
   int grid_tpp = 8;
   int map_layout = 0;
   int lbfgs_slots_per_cu = 8;
+  int score_ppb = 1;
 };
 Tuning g_tune;
 
@@ -703,6 +704,7 @@ static pxd_status grid_alloc(const pxd_receptor* r, const pxd_box* box, const in
   D.layout = 0;
   D.C = (long long)(box->npoints[0] - 1) * (box->npoints[1] - 1) * (box->npoints[2] - 1);
   D.cmaps = nullptr;
+  D.smaps = nullptr;
   g->cmem = nullptr;
   *out = g;
   return PXD_OK;
@@ -814,32 +816,36 @@ pxd_status pxd_grid_download(const pxd_grid* g, float* maps, size_t n_floats) {
 
 pxd_status pxd_grid_set_layout(pxd_grid* g, int32_t layout, void* stream) {
   if (!g) return fail(PXD_ERR_ARG, "grid layout: null grid");
-  if (layout != 0 && layout != 1) return fail(PXD_ERR_ARG, "grid layout: %d (0 or 1)", layout);
+  if (layout < 0 || layout > 2) return fail(PXD_ERR_ARG, "grid layout: %d (0, 1 or 2)", layout);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (layout == 1 && !g->cmem) {
-    size_t bytes = (size_t)(g->n_types + 1) * g->dev.C * 8 * sizeof(float4);
-    HIP_TRY(hipMalloc(&g->cmem, bytes));
-    launch_expand_cells(g->dev, g->n_types + 1, static_cast<float4*>(g->cmem), s);
+  if (layout == g->dev.layout) return PXD_OK;
+  // the lookup copy of the new layout (layouts 1 and 2; layout 0 reads the maps)
+  void* mem = nullptr;
+  if (layout != 0) {
+    const int nm = g->n_types + 1;
+    size_t bytes = layout == 1 ? (size_t)nm * g->dev.C * 8 * sizeof(float4)
+                               : (size_t)nm * 4 * g->dev.P * sizeof(float);
+    HIP_TRY(hipMalloc(&mem, bytes));
+    if (layout == 1)
+      launch_expand_cells(g->dev, nm, static_cast<float4*>(mem), s);
+    else
+      launch_split_planes(g->dev, nm, static_cast<float*>(mem), s);
     hipError_t he = hipGetLastError();
     if (he == hipSuccess) he = hipStreamSynchronize(s);
     if (he != hipSuccess) {
-      (void)hipFree(g->cmem);
-      g->cmem = nullptr;
+      (void)hipFree(mem);
       return fail(PXD_ERR_HIP, "grid layout: %s", hipGetErrorString(he));
     }
   }
   HIP_TRY(hipStreamSynchronize(s));  // no launch may still read the old descriptor
+  void* old = g->cmem;
+  g->cmem = mem;
   g->dev.layout = layout;
-  g->dev.cmaps = static_cast<const float4*>(g->cmem);
+  g->dev.cmaps = layout == 1 ? static_cast<const float4*>(mem) : nullptr;
+  g->dev.smaps = layout == 2 ? static_cast<const float*>(mem) : nullptr;
   pxd_status st = grid_sync_struct(g);
-  if (st != PXD_OK) return st;
-  if (layout == 0 && g->cmem) {
-    HIP_TRY(hipFree(g->cmem));
-    g->cmem = nullptr;
-    g->dev.cmaps = nullptr;
-    return grid_sync_struct(g);
-  }
-  return PXD_OK;
+  if (old) HIP_TRY(hipFree(old));
+  return st;
 }
 
 pxd_status pxd_grid_info(const pxd_grid* g, pxd_box* box, int32_t* n_types) {
@@ -1289,6 +1295,10 @@ pxd_status pxd_score_grad_batch(const pxd_receptor* r, const pxd_grid* g,
   if (st != PXD_OK) return st;
   ScoreArgs a{n_poses, pose_lig, pose_atom_off, xyz, tors, energy, lig_grad, tors_grad,
               b->max_atoms};
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    a.poses_per_block = g_tune.score_ppb;
+  }
   launch_score(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, static_cast<hipStream_t>(stream));
   HIP_TRY(hipGetLastError());
   return PXD_OK;
@@ -1382,20 +1392,25 @@ void pxd_get_tuning(pxd_tuning* out) {
   out->grid_types_per_pass = g_tune.grid_tpp;
   out->map_layout = g_tune.map_layout;
   out->lbfgs_slots_per_cu = g_tune.lbfgs_slots_per_cu;
+  out->score_poses_per_block = g_tune.score_ppb;
 }
 
 pxd_status pxd_set_tuning(const pxd_tuning* t) {
   if (!t) return fail(PXD_ERR_ARG, "tuning: null argument");
   if (t->grid_types_per_pass != 4 && t->grid_types_per_pass != 8 && t->grid_types_per_pass != 16)
     return fail(PXD_ERR_ARG, "tuning: types per pass %d (4, 8 or 16)", t->grid_types_per_pass);
-  if (t->map_layout != 0 && t->map_layout != 1)
-    return fail(PXD_ERR_ARG, "tuning: map layout %d (0 or 1)", t->map_layout);
+  if (t->map_layout < 0 || t->map_layout > 2)
+    return fail(PXD_ERR_ARG, "tuning: map layout %d (0, 1 or 2)", t->map_layout);
   if (t->lbfgs_slots_per_cu < 1 || t->lbfgs_slots_per_cu > 64)
     return fail(PXD_ERR_ARG, "tuning: slots per CU %d (1..64)", t->lbfgs_slots_per_cu);
+  if (t->score_poses_per_block != 1 && t->score_poses_per_block != 2 &&
+      t->score_poses_per_block != 4)
+    return fail(PXD_ERR_ARG, "tuning: poses per block %d (1, 2 or 4)", t->score_poses_per_block);
   std::lock_guard<std::mutex> lk(g_tune.mu);
   g_tune.grid_tpp = t->grid_types_per_pass;
   g_tune.map_layout = t->map_layout;
   g_tune.lbfgs_slots_per_cu = t->lbfgs_slots_per_cu;
+  g_tune.score_ppb = t->score_poses_per_block;
   return PXD_OK;
 }
 
@@ -1492,20 +1507,44 @@ pxd_status pxd_autotune(const pxd_receptor* r, const pxd_box* box, const int32_t
   HIP_TRY(hipMalloc(&e.p, 8 * (size_t)n_poses));
   HIP_TRY(hipMalloc(&gx.p, 12 * atoms));
   HIP_TRY(hipMalloc(&tg.p, 4 * (size_t)n_poses * (L + 1)));
+  auto time_score = [&](double* ms) {
+    return time_best(s, 3, [&]() {
+      return pxd_score_grad_batch(r, g, b, n_poses, pose_lig, pose_atom_off, xyz, tors,
+                                  (double*)e.p, (float*)gx.p, (float*)tg.p, stream);
+    }, ms);
+  };
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    g_tune.score_ppb = 1;
+  }
   best = 1e300;
-  for (int lay = 0; lay < 2 && st == PXD_OK; ++lay) {
+  for (int lay = 0; lay < 3 && st == PXD_OK; ++lay) {
     st = pxd_grid_set_layout(g, lay, stream);
-    if (st == PXD_OK)
-      st = time_best(s, 3, [&]() {
-        return pxd_score_grad_batch(r, g, b, n_poses, pose_lig, pose_atom_off, xyz, tors,
-                                    (double*)e.p, (float*)gx.p, (float*)tg.p, stream);
-      }, &t.score_ms[lay]);
+    if (st == PXD_OK) st = time_score(&t.score_ms[lay]);
     if (st == PXD_OK && t.score_ms[lay] < best) {
       best = t.score_ms[lay];
       t.map_layout = lay;
     }
   }
   if (st == PXD_OK) st = pxd_grid_set_layout(g, t.map_layout, stream);
+  // poses (waves) per workgroup of the batched score, on the chosen layout
+  const int ppbs[3] = {1, 2, 4};
+  best = 1e300;
+  for (int k = 0; k < 3 && st == PXD_OK; ++k) {
+    {
+      std::lock_guard<std::mutex> lk(g_tune.mu);
+      g_tune.score_ppb = ppbs[k];
+    }
+    st = time_score(&t.score_ppb_ms[k]);
+    if (st == PXD_OK && t.score_ppb_ms[k] < best) {
+      best = t.score_ppb_ms[k];
+      t.score_poses_per_block = ppbs[k];
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune.mu);
+    g_tune.score_ppb = t.score_poses_per_block;
+  }
   // 3. L-BFGS: resident work slots per CU (relax step, 40 iterations)
   const int spcs[4] = {2, 4, 8, 16};
   if (st == PXD_OK) {

@@ -89,6 +89,18 @@ __device__ inline float wave_max(float v) {
 
 __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
 
+// Ordering point of the per-pose code.  A pose is evaluated by one wave
+// whose lanes exchange data through LDS with lanes of the same wave only;
+// a wave's LDS (and global) accesses are performed in the order it issues
+// them, so no hardware barrier is needed -- the wavefront-scope fences keep
+// the compiler from moving accesses across the point.  Blocks that hold
+// several poses (one per wave) thus never wait for each other's waves.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------- geometry
 
 // math.h:176-178
@@ -512,7 +524,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, mov_ref(mm.x), r, -de);
     rec_grad_add(s, mov_ref(mm.y), r, de);
   }
-  __syncthreads();
+  wave_sync();
   if (tab) return e;  // movable-vs-fixed pairs are all inside the leaf tables
   // movable-vs-fixed pairs: per-slot segments padded to the wave width.  A
   // slot's gradient only enters through its leaf's dE/dtheta = jac . g, so
@@ -553,7 +565,7 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
 // atoms lane and lane + 64 into its registers g0 / g1 (fixed order: the sums
 // are deterministic).  gw: the chunk's gather word of this lane.
 __device__ inline void stage_gather(const PoseLds& s, uint32_t gw, float* g0, float* g1) {
-  __syncthreads();
+  wave_sync();
   const int o0 = gw & 255, c0 = (gw >> 8) & 255, o1 = (gw >> 16) & 255, c1 = gw >> 24;
   for (int k = 0; k < c0; ++k) {
     float4 v = s.stage[o0 + k];
@@ -567,7 +579,7 @@ __device__ inline void stage_gather(const PoseLds& s, uint32_t gw, float* g0, fl
     g1[1] += v.y;
     g1[2] += v.z;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 __device__ inline void stage_put(const PoseLds& s, uint32_t slots, int r, const float* v, float f) {
@@ -695,12 +707,12 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
   // stays in registers and is stored at the end of inter, so the region the
   // stage overlays is left to the ring's partner records
   if (li.pmode) {
-    __syncthreads();
+    wave_sync();
     return e;
   }
   // the stage is free again: clear the gradient array it overlays
   for (int k = lane; k < li.n_atoms * 3; k += kWave) s.grad[k] = 0.f;
-  __syncthreads();
+  wave_sync();
   for (int p = lane; p < li.n_pairs; p += kWave) {
     uint32_t w = L.pairs[li.pair_off + p];
     int i = w & 255, j = (w >> 8) & 255;
@@ -757,7 +769,7 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
     rec[2 * (a + n)] = r0;
     rec[2 * (a + n) + 1] = r1;
   }
-  __syncthreads();
+  wave_sync();
   // own factors of the pair parameters: q_a * kCoulomb, sigma_a / 2, 4 sqrt(eps_a)
   const float q0k = q0 * kCoulomb, sh0 = 0.5f * l0.x, e40 = 4.f * l0.y;
   const int nxt = act ? (a + 1 == n ? 0 : a + 1) : lane;  // ring successor (source of t)
@@ -942,6 +954,23 @@ __device__ inline float4 interp(const float4* map, const int* off, const float* 
   return acc;
 }
 
+// The same interpolation from component planes (map layout 2): value and
+// gradient components P floats apart, the same order of operations.
+__device__ inline float4 interp_planes(const float* map, long long P, const int* off,
+                                       const float* w) {
+  typedef const __attribute__((address_space(1))) float gf;
+  const gf* gm = (const gf*)map;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    acc.x += gm[off[c]] * w[c];
+    acc.y += gm[P + off[c]] * w[c];
+    acc.z += gm[2 * P + off[c]] * w[c];
+    acc.w += gm[3 * P + off[c]] * w[c];
+  }
+  return acc;
+}
+
 // receptor_cache::get (grid.cpp:197-262) for one ligand atom: adds into e/g.
 __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float* x,
                                 double* e, float* g) {
@@ -982,11 +1011,20 @@ __device__ inline void grid_get(const DevGrid& G, int slot, float q, const float
     hit = false;
   }
   // layout 0: point-major maps, the corners 1, nz, ny*nz points apart;
-  // layout 1: the cell's 8 corners contiguous (one 128-byte line per map).
+  // layout 1: the cell's 8 corners contiguous (one 128-byte line per map);
+  // layout 2: point-major component planes (value, dE/dx, dE/dy, dE/dz).
   // One base address per map; the corners are constant (layout 1) or shared
-  // 32-bit (layout 0) offsets from it.
+  // 32-bit (layouts 0, 2) offsets from it.
   float4 c, v;
-  if (G.layout) {
+  if (G.layout == 2) {
+    const int nz = G.np[2], nyz = G.np[1] * G.np[2];
+    const long long base = ((long long)lo[0] * G.np[1] + lo[1]) * nz + lo[2];
+    const float* p0 = G.smaps + base;
+    const float* p1 = G.smaps + 4 * G.P * slot + base;
+    const int o[8] = {0, nyz, nz, nyz + nz, 1, nyz + 1, nz + 1, nyz + nz + 1};
+    c = interp_planes(p0, G.P, o, w);
+    v = interp_planes(p1, G.P, o, w);
+  } else if (G.layout) {
     const long long cell =
         (((long long)lo[0] * (G.np[1] - 1) + lo[1]) * (G.np[2] - 1) + lo[2]) * 8;
     const float4* p0 = G.cmaps + cell;
@@ -1132,16 +1170,16 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
 // binding_system_interactions::put_gradients (interaction.cpp:271-298) for
 // the pose whose ligand coordinates and receptor torsions are in s.xyz /
 // s.tors.  Gradients are written to s.grad / s.torsg (overwrite).  Returns
-// the wave-uniform FP64 energy.  Uses __syncthreads(): one wave per block.
+// the wave-uniform FP64 energy.  One wave per pose (wave_sync: no block barrier).
 __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                                    const LigInfo& li, PoseLds& s) {
   const int lane = lane_id();
   for (int k = lane; k < R.n_leaves; k += kWave) s.torsg[k] = 0.f;
   receptor_torsion2xyz(R, s);
-  __syncthreads();
+  wave_sync();
   double e = receptor_intra(R, s);
   if (lane == 0) e += R.e_const;
-  __syncthreads();
+  wave_sync();
   float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
   e += ligand_intra(L, li, s, g0, g1);
   if (li.pmode) {
@@ -1151,7 +1189,7 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
       e += ligand_pairs_ring2(L, li, s, g0, g1);
   }
   e += inter(R, G, li, s, g0, g1);
-  __syncthreads();
+  wave_sync();
   // receptor_xyz2torsion (interaction.cpp:338-350): slots are grouped by leaf;
   // a leaf table contributes its energy and dT/dtheta directly
   for (int k = lane; k < R.n_leaves; k += kWave) {
@@ -1160,13 +1198,13 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
     e += te;
     s.torsg[k] += (float)td;
   }
-  __syncthreads();
+  wave_sync();
   for (int m = lane; m < R.n_mov; m += kWave) {
     float v = s.jac[3 * m] * s.movg[3 * m] + s.jac[3 * m + 1] * s.movg[3 * m + 1] +
               s.jac[3 * m + 2] * s.movg[3 * m + 2];
     atomicAdd(&s.torsg[R.mov_leaf[m]], v);
   }
-  __syncthreads();
+  wave_sync();
   return wave_sum(e);
 }
 

@@ -176,6 +176,9 @@ struct DevGrid {
   int layout;
   long long C;                // cells per map: (nx-1)(ny-1)(nz-1)
   const float4* cmaps;
+  // layout 2: point-major component planes, [(n_types+1)][4][P] floats
+  // (value, dE/dx, dE/dy, dE/dz)
+  const float* smaps;
 };
 
 }  // namespace pxd
@@ -203,7 +206,7 @@ struct pxd_receptor {
 
 struct pxd_grid {
   pxd::DevGrid dev;
-  void* cmem;                 // cell-corner maps of layout 1 (nullptr: not built)
+  void* cmem;                 // lookup copy of layout 1 or 2 (nullptr: layout 0)
   pxd_box box;
   int n_types;
   int32_t types[PXD_NUM_VDW_TYPES];

@@ -286,23 +286,24 @@ static_assert(pose_dims(kMaxAtoms, kMaxMovable, kMaxLeaves).total == PXD_POSE_LD
               "pose LDS layout at the maxima");
 
 // ============================================================== K2: score+grad
-// One wavefront (= one block) per pose.
+// One wavefront per pose; W poses (waves) per block, each wave with its own
+// pose arrays in the block's dynamic LDS.  The per-pose code synchronises
+// within its wave only (wave_sync), so the waves of a block are independent.
 
 // Occupancy target 6 waves per SIMD: the compiler then schedules within
 // 85 VGPRs (77 used, no spills); measured 1.91 ms at the default 4 waves
 // (101 VGPRs), 1.70 ms at 6, 1.74 ms at 5 and 7 on the bench batch.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_score(DevReceptor R, DevLigands L,
-                                              const DevGrid* __restrict__ G, int n_poses,
-                                              const int* __restrict__ pose_lig,
-                                              const int* __restrict__ pose_off,
-                                              const float* __restrict__ xyz,
-                                              const float* __restrict__ tors,
-                                              double* __restrict__ energy,
-                                              float* __restrict__ lig_grad,
-                                              float* __restrict__ tors_grad, int max_atoms) {
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(6))) void k_score(
+    DevReceptor R, DevLigands L, const DevGrid* __restrict__ G, int n_poses,
+    const int* __restrict__ pose_lig, const int* __restrict__ pose_off,
+    const float* __restrict__ xyz, const float* __restrict__ tors, double* __restrict__ energy,
+    float* __restrict__ lig_grad, float* __restrict__ tors_grad, int max_atoms) {
   // pose arrays sized for this launch's largest ligand and this receptor
-  PoseLds s = carve_pose_lds(dyn_lds, pose_dims(max_atoms, R.n_mov, R.n_leaves));
-  const int p = xcd_block(blockIdx.x, gridDim.x);
+  const PoseDims dims = pose_dims(max_atoms, R.n_mov, R.n_leaves);
+  const int wv = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+  PoseLds s = carve_pose_lds(dyn_lds + wv * dims.total, dims);
+  const int p = xcd_block(blockIdx.x, gridDim.x) * W + wv;
   if (p >= n_poses) return;
   const int lane = lane_id();
   const LigInfo li = L.info[pose_lig[p]];
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
   for (int k = lane; k < li.n_atoms * 3; k += kWave) s.xyz[k] = xyz[off + k];
   for (int k = lane; k < R.n_leaves; k += kWave)
     s.tors[k] = tors ? tors[(long long)p * R.n_leaves + k] : 0.f;
-  __syncthreads();
+  wave_sync();
   double e = eval_pose(R, L, G, li, s);
   if (lane == 0) energy[p] = e;
   for (int k = lane; k < li.n_atoms * 3; k += kWave) lig_grad[off + k] = s.grad[k];
@@ -320,14 +321,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
       tors_grad[(long long)p * R.n_leaves + k] = s.torsg[k];
 }
 
+template <int W>
+static void launch_score_w(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                           const ScoreArgs& a, int ma, hipStream_t st) {
+  const size_t lds = sizeof(float) * W * pose_dims(ma, R.n_mov, R.n_leaves).total;
+  const int blocks = (a.n_poses + W - 1) / W;
+  hipLaunchKernelGGL(k_score<W>, dim3(blocks), dim3(kWave * W), lds, st, R, L, G, a.n_poses,
+                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.energy, a.lig_grad, a.tors_grad,
+                     ma);
+}
+
 void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const ScoreArgs& a, hipStream_t st) {
   if (a.n_poses <= 0) return;
   const int ma = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
-  const size_t lds = sizeof(float) * pose_dims(ma, R.n_mov, R.n_leaves).total;
-  hipLaunchKernelGGL(k_score, dim3(a.n_poses), dim3(kWave), lds, st, R, L, G, a.n_poses,
-                     a.pose_lig, a.pose_off, a.xyz, a.tors, a.energy, a.lig_grad, a.tors_grad,
-                     ma);
+  if (a.poses_per_block == 4)
+    launch_score_w<4>(R, L, G, a, ma, st);
+  else if (a.poses_per_block == 2)
+    launch_score_w<2>(R, L, G, a, ma, st);
+  else
+    launch_score_w<1>(R, L, G, a, ma, st);
 }
 
 // ============================================================== K3: L-BFGS
@@ -1579,6 +1592,28 @@ __global__ __launch_bounds__(256) void k_expand_cells(const float4* __restrict__
   const int i = (int)(cell / ((long long)(nz - 1) * (ny - 1)));
   const long long p = ((long long)(i + (c & 1)) * ny + (j + ((c >> 1) & 1))) * nz + k + (c >> 2);
   cm[t] = maps[(long long)m * P + p];
+}
+
+// ============================================================== map layout 2
+// Splits every map's float4 points into four planes:
+// sp[(m*4 + c)*P + p] = component c of maps[m*P + p].
+__global__ __launch_bounds__(256) void k_split_planes(const float4* __restrict__ maps, long long P,
+                                                      int n_maps, float* __restrict__ sp) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * n_maps) return;
+  const long long m = t / P, p = t - m * P;
+  const float4 v = maps[t];
+  float* o = sp + m * 4 * P + p;
+  o[0] = v.x;
+  o[P] = v.y;
+  o[2 * P] = v.z;
+  o[3 * P] = v.w;
+}
+
+void launch_split_planes(const DevGrid& G, int n_maps, float* sp, hipStream_t st) {
+  const long long total = G.P * n_maps;
+  hipLaunchKernelGGL(k_split_planes, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     G.maps, G.P, n_maps, sp);
 }
 
 void launch_expand_cells(const DevGrid& G, int n_maps, float4* cm, hipStream_t st) {

@@ -38,6 +38,7 @@ struct ScoreArgs {
   float* lig_grad;
   float* tors_grad;
   int max_atoms;  // largest ligand of the batch (sizes the per-pose LDS); <= 0: the maximum
+  int poses_per_block = 1;  // poses (waves) per workgroup: 1, 2 or 4
 };
 
 struct LbfgsParamsDev {
@@ -211,6 +212,7 @@ void launch_leaf_table(const DevReceptor& R, int n, double shift, double2* out,
                        hipStream_t st);
 // Map layout 1: every cell's 8 corners of every map stored together.
 void launch_expand_cells(const DevGrid& G, int n_maps, float4* cm, hipStream_t st);
+void launch_split_planes(const DevGrid& G, int n_maps, float* sp, hipStream_t st);
 // Rigid-body + torsion gradient of poses from their Cartesian gradient:
 // out[p][6 + kMaxFrags - 1] = force, torque about the centre, torsion torques.
 void launch_dof_grad(const DevLigands& L, int n_poses, const int* pose_lig, const int* pose_off,

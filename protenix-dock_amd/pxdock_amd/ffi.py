@@ -183,15 +183,18 @@ def mc_params(seed=0, max_nsteps=200, relax_iters=40, full_iters=240, steps_per_
 class PxdTuning(C.Structure):
     _fields_ = [
         ("grid_types_per_pass", C.c_int32), ("map_layout", C.c_int32),
-        ("lbfgs_slots_per_cu", C.c_int32), ("grid_ms", C.c_double * 3),
-        ("score_ms", C.c_double * 2), ("lbfgs_ms", C.c_double * 4),
+        ("lbfgs_slots_per_cu", C.c_int32), ("score_poses_per_block", C.c_int32),
+        ("grid_ms", C.c_double * 3), ("score_ms", C.c_double * 3),
+        ("lbfgs_ms", C.c_double * 4), ("score_ppb_ms", C.c_double * 3),
     ]
 
     def as_dict(self):
         return {"grid_types_per_pass": self.grid_types_per_pass, "map_layout": self.map_layout,
                 "lbfgs_slots_per_cu": self.lbfgs_slots_per_cu,
+                "score_poses_per_block": self.score_poses_per_block,
                 "grid_ms": {k: v for k, v in zip((4, 8, 16), self.grid_ms)},
-                "score_ms": {k: v for k, v in zip((0, 1), self.score_ms)},
+                "score_ms": {k: v for k, v in zip((0, 1, 2), self.score_ms)},
+                "score_ppb_ms": {k: v for k, v in zip((1, 2, 4), self.score_ppb_ms)},
                 "lbfgs_ms": {k: v for k, v in zip((2, 4, 8, 16), self.lbfgs_ms)}}
 
 

@@ -587,11 +587,13 @@ def get_tuning():
     return t
 
 
-def set_tuning(grid_types_per_pass=8, map_layout=0, lbfgs_slots_per_cu=8):
+def set_tuning(grid_types_per_pass=8, map_layout=0, lbfgs_slots_per_cu=8,
+               score_poses_per_block=1):
     t = ffi.PxdTuning()
     t.grid_types_per_pass = grid_types_per_pass
     t.map_layout = map_layout
     t.lbfgs_slots_per_cu = lbfgs_slots_per_cu
+    t.score_poses_per_block = score_poses_per_block
     check(load().pxd_set_tuning(_ref(t)))
 
 

@@ -45,13 +45,17 @@ def test_error_channel_and_host_helpers():
 
 def test_tuning_defaults_and_validation():
     t = lib.get_tuning()
-    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu) == (8, 0, 8)
-    for bad in ({"grid_types_per_pass": 5}, {"map_layout": 2}, {"lbfgs_slots_per_cu": 0}):
+    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu,
+            t.score_poses_per_block) == (8, 0, 8, 1)
+    for bad in ({"grid_types_per_pass": 5}, {"map_layout": 3}, {"map_layout": -1},
+                {"lbfgs_slots_per_cu": 0}, {"score_poses_per_block": 3},
+                {"score_poses_per_block": 8}):
         with pytest.raises(lib.PxdError):
             lib.set_tuning(**bad)
-    lib.set_tuning(16, 1, 4)
+    lib.set_tuning(16, 2, 4, 4)
     t = lib.get_tuning()
-    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu) == (16, 1, 4)
+    assert (t.grid_types_per_pass, t.map_layout, t.lbfgs_slots_per_cu,
+            t.score_poses_per_block) == (16, 2, 4, 4)
     lib.set_tuning()
 
 

@@ -443,20 +443,29 @@ def test_gpu_lbfgs_rigid_torsion_matches_oracle(world, iters, need):
 
 
 def test_gpu_map_layouts_give_identical_results(world):
-    """Map layout 1 (cell corners together) reads the same values in the same
-    order as layout 0: energies and gradients are bit-identical."""
+    """Map layouts 1 (cell corners together) and 2 (component planes) read the
+    same values in the same order as layout 0, and 2 or 4 poses per workgroup
+    evaluate each pose as one: energies and gradients are bit-identical.
+    The pose count (23) is not a multiple of 2 or 4, so the last block holds
+    waves without a pose."""
     pose_lig, xyz, tors = _poses(world, 6, 71)
+    pose_lig, xyz, tors = pose_lig[:-1], xyz[:-1], tors[:-1]
     B = lib.LigandBatch(world["ligs"])
     P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    assert P.n_poses % 2 != 0
     e0, g0, t0 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
-    world["G"].set_layout(1)
     try:
-        e1, g1, t1 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+        for layout in (1, 2):
+            world["G"].set_layout(layout)
+            for ppb in (1, 2, 4):
+                lib.set_tuning(8, 0, 8, ppb)
+                e1, g1, t1 = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+                np.testing.assert_array_equal(e1, e0, err_msg=f"layout {layout} ppb {ppb}")
+                np.testing.assert_array_equal(g1, g0, err_msg=f"layout {layout} ppb {ppb}")
+                np.testing.assert_array_equal(t1, t0, err_msg=f"layout {layout} ppb {ppb}")
     finally:
+        lib.set_tuning()
         world["G"].set_layout(0)
-    np.testing.assert_array_equal(e1, e0)
-    np.testing.assert_array_equal(g1, g0)
-    np.testing.assert_array_equal(t1, t0)
 
 
 def test_gpu_grid_build_variants_identical_and_autotune(world):
@@ -479,10 +488,15 @@ def test_gpu_grid_build_variants_identical_and_autotune(world):
         t = lib.autotune(world["R"], world["box"], world["types"], P)
         d = t.as_dict()
         print("\nautotune:", d)
-        assert d["grid_types_per_pass"] in (4, 8, 16) and d["map_layout"] in (0, 1)
+        assert d["grid_types_per_pass"] in (4, 8, 16) and d["map_layout"] in (0, 1, 2)
         assert d["lbfgs_slots_per_cu"] in (2, 4, 8, 16)
+        assert d["score_poses_per_block"] in (1, 2, 4)
         assert all(v > 0 for v in d["grid_ms"].values())
         assert all(v > 0 for v in d["score_ms"].values())
+        assert all(v > 0 for v in d["score_ppb_ms"].values())
+        # every choice is the fastest of its measured variants
+        assert d["score_ppb_ms"][d["score_poses_per_block"]] == min(d["score_ppb_ms"].values())
+        assert d["score_ms"][d["map_layout"]] == min(d["score_ms"].values())
         assert all(v > 0 for v in d["lbfgs_ms"].values())
         np.testing.assert_array_equal(P.xyz.cpu().numpy(), x_before)  # inputs untouched
     finally:
