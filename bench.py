@@ -150,6 +150,20 @@ def grid_flops(rec, n_points, n_types):
     return float(n_points) * n_fixed * (26.0 + 30.0 * n_types)
 
 
+def lbfgs_roofline(st, seconds):
+    """HBM roofline of a batched L-BFGS run from the library's work totals
+    (pxd_lbfgs_last_stats): its algorithmic bytes are the two-loop history
+    traffic (every stored s/y pair read twice per iteration, n floats each,
+    plus the writes) and the map lookups (256 B per ligand atom and
+    evaluation), over the call's wall time."""
+    b = st["history_bytes"] + st["map_bytes"]
+    return {"bound": "hbm", "achieved": b / seconds / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": b / seconds / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes": b,
+            "history_bytes": st["history_bytes"], "map_bytes": st["map_bytes"],
+            "iterations": st["iterations"], "evaluations": st["evaluations"],
+            "kernel": "k_lbfgs"}
+
+
 def _make_search_ligand(i):
     from pxdock_amd import synth
     rng = np.random.default_rng(30000 + i + 7919)
@@ -284,6 +298,7 @@ def large_leg(args, workers):
     _, ne, cv = lib.lbfgs(R, G, P, prm)
     torch.cuda.synchronize()
     dt = time.time() - t1
+    lroof = lbfgs_roofline(lib.lbfgs_last_stats(), dt)
     cv, ne = cv.cpu().numpy(), ne.cpu().numpy()
     d = {"receptor_heavy_atoms": 8000, "receptor_atoms": int(rec.n_atoms), "box_A": 40.0,
          "grid_points": box.npts, "maps": len(types) + 1,
@@ -293,7 +308,7 @@ def large_leg(args, workers):
          "lbfgs": {"ligands": len(ligs), "starts_per_ligand": per, "poses": int(P.n_poses),
                    "seconds": dt, "converged_frac": float(cv.mean()),
                    "converged_poses_per_s": float(cv.sum() / dt),
-                   "evals_per_s": float(ne.sum() / dt)},
+                   "evals_per_s": float(ne.sum() / dt), "roofline": lroof},
          "sample": f"{len(ligs)} of the 65,536 screening ligands x {per} starts"}
     log(f"large: {d}")
     return d
@@ -603,7 +618,8 @@ def main():
         le = le.cpu().numpy()
         batched = {"poses": nl, "seconds": dt, "converged_frac": float(lcv.mean()),
                    "converged_poses_per_s": float(lcv.sum() / dt),
-                   "evals_per_s": float(lne.sum() / dt)}
+                   "evals_per_s": float(lne.sum() / dt),
+                   "roofline": lbfgs_roofline(lib.lbfgs_last_stats(), dt)}
         # the plain form on an evenly strided sample of the same poses
         npl = min(args.plain_poses, nl)
         ps = np.linspace(0, nl - 1, npl).astype(int)

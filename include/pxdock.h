@@ -424,6 +424,19 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* grid,
                            double* energy, int32_t* nevals, int32_t* converged,
                            void* stream);
 
+/* Work totals of the most recent pxd_lbfgs_batch call in this process (all
+ * zero before the first).  No counterpart in the reference; they give the
+ * algorithmic bytes of a run for its roofline.  A history entry is the n
+ * floats of s and of y plus its FP64 1 / (s.y), n the pose's variable count;
+ * the two-loop recursion reads every stored entry twice per iteration. */
+typedef struct pxd_lbfgs_stats {
+  uint64_t iterations;    /* L-BFGS iterations, all poses */
+  uint64_t evaluations;   /* score+grad evaluations, all poses */
+  uint64_t history_bytes; /* history entries read and written x (8 n + 8) bytes */
+  uint64_t map_bytes;     /* map lookups: 256 bytes per ligand atom and evaluation */
+} pxd_lbfgs_stats;
+pxd_status pxd_lbfgs_last_stats(pxd_lbfgs_stats* out);
+
 /* pxd_lbfgs_batch for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * No counterpart in the reference.  Same arguments, results and checks as

@@ -209,6 +209,7 @@ struct LbfgsWorkspace {
   float* hist = nullptr;
   size_t hist_floats = 0;
   int* counter = nullptr;
+  pxd_lbfgs_stats last{};  // totals of the last pxd_lbfgs_batch
 };
 LbfgsWorkspace g_ws;
 
@@ -1270,6 +1271,7 @@ pxd_status lbfgs_prepare(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   a->counter = nullptr;
   a->hist = nullptr;
   a->max_atoms = b->max_atoms;
+  a->stats = nullptr;
   return PXD_OK;
 }
 
@@ -1324,13 +1326,25 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
     g_ws.hist_floats = per_slot * n_slots;
   }
   if (!g_ws.counter) HIP_TRY(hipMalloc(&g_ws.counter, sizeof(int) * 64));
-  HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int) * kLbfgsClasses, s));
+  // queue heads at ints 0..2, the statistics (4 x uint64) at ints 16..23
+  HIP_TRY(hipMemsetAsync(g_ws.counter, 0, sizeof(int) * 24, s));
   a.counter = g_ws.counter;
   a.hist = g_ws.hist;
+  a.stats = reinterpret_cast<unsigned long long*>(g_ws.counter + 16);
   launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
   HIP_TRY(hipGetLastError());
   // the shared workspace may be reused by the next call: finish this one
   HIP_TRY(hipStreamSynchronize(s));
+  unsigned long long st4[4];
+  HIP_TRY(hipMemcpy(st4, a.stats, sizeof(st4), hipMemcpyDeviceToHost));
+  g_ws.last = {st4[0], st4[1], st4[2], st4[3]};
+  return PXD_OK;
+}
+
+pxd_status pxd_lbfgs_last_stats(pxd_lbfgs_stats* out) {
+  if (!out) return fail(PXD_ERR_ARG, "lbfgs stats: null argument");
+  std::lock_guard<std::mutex> lock(g_ws.mu);
+  *out = g_ws.last;
   return PXD_OK;
 }
 

@@ -366,6 +366,8 @@ struct LbfgsCtx {
   int n3, n;  // n3: ligand variables (3 N, or 6 + n_tors), n: all variables
   int nk;     // vector elements per lane in use: ceil(n / 64) <= VPL
   int nevals;
+  int n_iter;          // iterations of the last lbfgs_solve
+  long long hist_io;   // history entries read (two-loop) and written by it
   // rigid-body + torsion variables: the input pose, its centre, LDS scratch
   int mode;
   float* xyz0;   // [kMaxAtoms*3]
@@ -655,6 +657,8 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
                             bool* converged_out, bool evaluate_only = false) {
   const int lane = lane_id();
   const int m = P.history;
+  c.n_iter = 0;
+  c.hist_io = 0;
   // history entries are stored at the class's full width (VPL elements per
   // lane; m entries fill the caller's history*64*VPL floats of hy and of hs)
   constexpr int NV = kWave * VPL;
@@ -676,6 +680,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
     converged = true;
   } else {
     int head = 0, num_old = 0, n_iter = 0;
+    long long hist_io = 0;
     bool high_change = false;
     double H = 1.0, t = 0.0, prev_loss;
     while (n_iter < P.max_iter && c.nevals < P.max_eval) {
@@ -715,6 +720,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             hs[(size_t)slot * NV + lane + kWave * k] = in ? sv[lane + kWave * k] : 0.f;
           }
           hr[slot] = 1.0 / ys;  // every lane stores (and later reads) the same value
+          hist_io += 1;
           __syncthreads();
         }
         // two-loop recursion (optimizer.cpp:493-507), its dot products in FP32
@@ -729,6 +735,7 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
         // The arithmetic does not depend on kPre or VPL: every size class
         // gives the same result for the same pose.
         constexpr int kPre = VPL <= 2 ? PXD_LBFGS_PRE2 : (VPL <= 4 ? PXD_LBFGS_PRE4 : PXD_LBFGS_PRE7);
+        hist_io += 2 * num_old;  // each loop reads every stored entry once
         float qr[VPL], bs[kPre][VPL], by[kPre][VPL];
         double br[kPre];
 #pragma unroll
@@ -846,6 +853,8 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
       for (int k = 0; k < VPL; ++k) if (k < c.nk) d[lane + kWave * k] = 0.f;
       energy = eval_dir(c, 0.0, d, g);
     }
+    c.n_iter = n_iter;
+    c.hist_io = hist_io;
   }
   *energy_out = energy;
   *converged_out = converged;
@@ -934,6 +943,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
       A.energy[pi] = energy;
       A.nevals[pi] = c.nevals;
       A.converged[pi] = converged ? 1 : 0;
+      if (A.stats) {
+        // a history entry is n floats of s and of y plus its FP64 1 / (s.y)
+        atomicAdd(&A.stats[0], (unsigned long long)c.n_iter);
+        atomicAdd(&A.stats[1], (unsigned long long)c.nevals);
+        atomicAdd(&A.stats[2], (unsigned long long)c.hist_io * (8ull * c.n + 8ull));
+        atomicAdd(&A.stats[3], (unsigned long long)c.nevals * 256ull * c.li.n_atoms);
+      }
     }
     __syncthreads();
   }

@@ -60,6 +60,9 @@ struct LbfgsArgs {
   int* counter;   // work-queue heads (one per size class), zero before launch
   float* hist;    // [n_slots][lbfgs_slot_floats(history)]
   int max_atoms;  // largest ligand of the batch (sizes the per-pose LDS); <= 0: the maximum
+  // nullable, zero before launch: totals over the poses of iterations,
+  // evaluations, history bytes and map-lookup bytes (pxd_lbfgs_stats)
+  unsigned long long* stats;
 };
 
 // Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each,

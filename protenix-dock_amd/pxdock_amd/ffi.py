@@ -180,6 +180,14 @@ def mc_params(seed=0, max_nsteps=200, relax_iters=40, full_iters=240, steps_per_
     return p
 
 
+class PxdLbfgsStats(C.Structure):
+    _fields_ = [("iterations", C.c_uint64), ("evaluations", C.c_uint64),
+                ("history_bytes", C.c_uint64), ("map_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class PxdTuning(C.Structure):
     _fields_ = [
         ("grid_types_per_pass", C.c_int32), ("map_layout", C.c_int32),

@@ -71,6 +71,7 @@ def load():
         "pxd_pscore_grad_batch": (i, [P, P, P, i, P, P, P, P, P, P, P, P]),
         "pxd_get_tuning": (None, [P]),
         "pxd_set_tuning": (i, [P]),
+        "pxd_lbfgs_last_stats": (i, [P]),
         "pxd_autotune": (i, [P, P, i32p, i, P, i, P, P, P, P, P, P]),
         "pxd_mc_search": (i, [P, P, P, P, P, P, i, P, P, P, P, P, P, i, d, P, P, P]),
         "pxd_mc_search_conformers": (i, [P, P, P, P, P, P, i, i32p, P, P, P, P, P, P, i, d, P,
@@ -369,6 +370,13 @@ def lbfgs(receptor, grid, poses, params):
                                  _dev(poses.xyz), _dev(poses.tors), _dev(e), _dev(ne), _dev(cv),
                                  _stream()))
     return e, ne, cv
+
+
+def lbfgs_last_stats():
+    """pxd_lbfgs_last_stats: work totals of the last lbfgs() call (dict)."""
+    st = ffi.PxdLbfgsStats()
+    check(load().pxd_lbfgs_last_stats(_ref(st)))
+    return st.as_dict()
 
 
 def lbfgs_noalloc(receptor, grid, poses, params, out=None):

@@ -501,3 +501,36 @@ def test_gpu_grid_build_variants_identical_and_autotune(world):
         np.testing.assert_array_equal(P.xyz.cpu().numpy(), x_before)  # inputs untouched
     finally:
         lib.set_tuning()
+
+
+def test_gpu_lbfgs_stats_count_the_work(world):
+    """pxd_lbfgs_last_stats: evaluations equal the per-pose counts; map bytes
+    are 256 B per ligand atom and evaluation; iterations are at most the
+    evaluations after each pose's first; the history traffic is a whole
+    number of (8 n + 8)-byte entries, none with one iteration, and at most
+    2 m + 1 entries per iteration."""
+    l = 0  # poses of one ligand: one n for all of them
+    lig = world["ligs"][l]
+    pose_lig, xyz, tors = _poses(world, 12, 93, spread=2.0)
+    keep = [i for i, pl in enumerate(pose_lig) if pl == l]
+    pose_lig = [pose_lig[i] for i in keep]
+    xyz = [xyz[i] for i in keep]
+    tors = tors[keep]
+    B = lib.LigandBatch(world["ligs"])
+    n = 3 * lig.n_atoms + 8
+    entry = 8 * n + 8
+    for max_iter, m in ((1, 100), (30, 4), (60, 100)):
+        P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+        e, ne, cv = lib.lbfgs(world["R"], world["G"], P, ffi.lbfgs_params(max_iter, history=m))
+        st = lib.lbfgs_last_stats()
+        ne = ne.cpu().numpy().astype(np.int64)
+        print(f"\nmax_iter {max_iter} history {m}: {st}")
+        assert st["evaluations"] == int(ne.sum())
+        assert st["map_bytes"] == int(ne.sum()) * 256 * lig.n_atoms
+        assert 0 <= st["iterations"] <= int(ne.sum()) - len(ne)
+        assert st["history_bytes"] % entry == 0
+        assert st["history_bytes"] <= st["iterations"] * (2 * m + 1) * entry
+        if max_iter == 1:
+            assert st["history_bytes"] == 0
+        else:
+            assert st["history_bytes"] > 0 and st["iterations"] > len(ne)
