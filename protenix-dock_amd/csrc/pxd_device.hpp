@@ -283,14 +283,14 @@ __device__ inline float pair_energy_qk(float rx, float ry, float rz, float qk, f
   float rd = fminf(__builtin_amdgcn_rsqf(d2), 1e20f);
   float rd2 = rd * rd;
   float ec = qk * rd;
-  float dc = -qk * rd2 * rd;
   float u = sig * rd;
   float u2 = u * u;
   float u6 = u2 * u2 * u2;
   float u12 = u6 * u6;
-  float dv = (-12.f * u12 + 6.f * u6) * rd2 * e4;
-  *de_dr_r = dc + dv;
-  return ec + (u12 - u6) * e4;
+  // dE/dd / d = (-qk / d^3) + (6 u^6 - 12 u^12) e4 / d^2, with 1/d^2 taken out
+  const float w = fmaf(-12.f, u12, 6.f * u6);
+  *de_dr_r = rd2 * fmaf(w, e4, -ec);
+  return fmaf(u12 - u6, e4, ec);
 }
 
 __device__ inline float pair_energy(float rx, float ry, float rz, float qq, float sig,
@@ -776,12 +776,17 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   const int K = n / 2;
   float t[3] = {0.f, 0.f, 0.f}, es = 0.f;
   const float4* pr = rec + 2 * (a + 1);  // partner a + k's record in step k
-  // bits 1 .. 32 of the rotated masks (K <= 32 steps), bit 0 = step 1
+  // bits 1 .. 32 of the rotated masks (K <= 32 steps), bit k - 1 = step k;
+  // with n even, the pair (a, a + n/2) is met from a < n/2 only: lanes
+  // a >= n/2 drop step n/2 (inactive lanes have empty masks)
   uint32_t ba = (uint32_t)(ra >> 1), bf = (uint32_t)(rf >> 1);
-  for (int k = 1; k <= K; ++k, pr += 2, ba >>= 1, bf >>= 1) {
-    const uint32_t ia = ba & 1u, i4 = bf & 1u;
-    const bool live = act && (ia | i4) && !(2 * k == n && a >= k);
-    if (live) {
+  if (K > 0 && 2 * K == n && a >= K) {
+    ba &= ~(1u << (K - 1));
+    bf &= ~(1u << (K - 1));
+  }
+  for (int k = 1; k <= K; ++k, pr += 2) {
+    const uint32_t ia = __builtin_amdgcn_ubfe(ba, k - 1, 1), i4 = __builtin_amdgcn_ubfe(bf, k - 1, 1);
+    if (ia | i4) {
       const float fa = (float)ia, f4 = (float)i4;  // scales: 1, the 1-4 scale, or their sum
       const float cs = fmaf(f4, kScale14Coul, fa), vs = fmaf(f4, kScale14Vdw, fa);
       const float4 pb = pr[0];
@@ -854,14 +859,35 @@ __device__ inline double ligand_pairs_ring2(const DevLigands& L, const LigInfo& 
   const bool last1 = i1 + 1 == n;            // slot 1's last owner: its successor is owner 0
   float t0[3] = {0.f, 0.f, 0.f}, t1[3] = {0.f, 0.f, 0.f}, es = 0.f;
   double e = 0.0;
+  // with n even, the pair (i, i + n/2) is met from i < n/2 only: owners
+  // i >= n/2 drop step n/2 (bit K - 1; inactive slot 1 has empty masks)
+  if (2 * K == n) {
+    const uint64_t drop = ~(1ull << (K - 1));
+    if (i0 >= K) {
+      a0 &= drop;
+      f0 &= drop;
+    }
+    if (i1 >= K) {
+      a1 &= drop;
+      f1 &= drop;
+    }
+  }
+  // the 32-bit half of the masks that holds step k's bit (k - 1)
+  uint32_t wa0 = (uint32_t)a0, wf0 = (uint32_t)f0, wa1 = (uint32_t)a1, wf1 = (uint32_t)f1;
   int b0 = i0, b1 = j1;
   for (int k = 1; k <= K; ++k) {
     b0 = b0 + 1 == n ? 0 : b0 + 1;
     b1 = b1 + 1 == n ? 0 : b1 + 1;
-    const bool even_last = 2 * k == n;  // the pair (i, i + n/2) only from i < n/2
+    if (k == 33) {
+      wa0 = (uint32_t)(a0 >> 32);
+      wf0 = (uint32_t)(f0 >> 32);
+      wa1 = (uint32_t)(a1 >> 32);
+      wf1 = (uint32_t)(f1 >> 32);
+    }
+    const int bit = (k - 1) & 31;
     {
-      const bool ia = a0 & 1ull, i4 = f0 & 1ull;
-      if ((ia || i4) && !(even_last && i0 >= k)) {
+      const bool ia = __builtin_amdgcn_ubfe(wa0, bit, 1), i4 = __builtin_amdgcn_ubfe(wf0, bit, 1);
+      if (ia || i4) {
         const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
         const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
         const float2 lb = s.lj[b0];
@@ -879,8 +905,8 @@ __device__ inline double ligand_pairs_ring2(const DevLigands& L, const LigInfo& 
       }
     }
     {
-      const bool ia = a1 & 1ull, i4 = f1 & 1ull;  // zero masks when slot 1 is inactive
-      if ((ia || i4) && !(even_last && i1 >= k)) {
+      const bool ia = __builtin_amdgcn_ubfe(wa1, bit, 1), i4 = __builtin_amdgcn_ubfe(wf1, bit, 1);
+      if (ia || i4) {  // zero masks when slot 1 is inactive
         const float cs = (ia ? 1.f : 0.f) + (i4 ? kScale14Coul : 0.f);
         const float vs = (ia ? 1.f : 0.f) + (i4 ? kScale14Vdw : 0.f);
         const float2 lb = s.lj[b1];
@@ -897,10 +923,6 @@ __device__ inline double ligand_pairs_ring2(const DevLigands& L, const LigInfo& 
         t1[2] += rz * de;
       }
     }
-    a0 >>= 1;
-    f0 >>= 1;
-    a1 >>= 1;
-    f1 >>= 1;
     if ((k & 31) == 0) {  // FP32 pair energies per block of 32 steps, then FP64
       e += (double)es;
       es = 0.f;
