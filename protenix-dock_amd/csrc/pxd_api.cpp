@@ -210,6 +210,11 @@ struct LbfgsWorkspace {
   size_t hist_floats = 0;
   int* counter = nullptr;
   pxd_lbfgs_stats last{};  // totals of the last pxd_lbfgs_batch
+  int* order = nullptr;    // poses grouped by size class
+  size_t order_n = 0;
+  // side streams (default priority) and events of the concurrent class launches
+  hipStream_t side[kLbfgsClasses - 1] = {};
+  hipEvent_t fork = nullptr, join[kLbfgsClasses - 1] = {};
 };
 LbfgsWorkspace g_ws;
 
@@ -1272,6 +1277,8 @@ pxd_status lbfgs_prepare(const pxd_receptor* r, const pxd_grid* g, const pxd_lig
   a->hist = nullptr;
   a->max_atoms = b->max_atoms;
   a->stats = nullptr;
+  a->order = nullptr;
+  a->n_order = 0;
   return PXD_OK;
 }
 
@@ -1318,6 +1325,39 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   hipStream_t s = static_cast<hipStream_t>(stream);
   size_t per_slot = lbfgs_slot_floats(prm->history);
   std::lock_guard<std::mutex> lock(g_ws.mu);
+  // Cartesian variables: the poses grouped by size class on the host (stable
+  // order), one launch per class, the classes side by side on streams of
+  // their own so that one class's tail overlaps the others' work; work
+  // slots shared out by each class's poses x vector width.
+  int cls_n[kLbfgsClasses] = {0, 0, 0}, cls_slots[kLbfgsClasses] = {0, 0, 0};
+  std::vector<int32_t> order;
+  if (!prm->dof_mode) {
+    HIP_TRY(hipStreamSynchronize(s));  // pose_lig may be written by earlier work on s
+    std::vector<int32_t> hl(n_poses);
+    HIP_TRY(hipMemcpy(hl.data(), pose_lig, 4 * (size_t)n_poses, hipMemcpyDeviceToHost));
+    std::vector<int8_t> cls(n_poses);
+    for (int p = 0; p < n_poses; ++p) {
+      if (hl[p] < 0 || hl[p] >= b->n) return fail(PXD_ERR_ARG, "lbfgs: pose ligand out of range");
+      const int n = 3 * b->h_natoms[hl[p]] + r->n_leaves;
+      const int nk = (n + kWave - 1) / kWave;
+      cls[p] = nk <= kLbfgsClassVpl[0] ? 0 : (nk <= kLbfgsClassVpl[1] ? 1 : 2);
+      cls_n[cls[p]]++;
+    }
+    order.reserve(n_poses);
+    for (int c = 0; c < kLbfgsClasses; ++c)
+      for (int p = 0; p < n_poses; ++p)
+        if (cls[p] == c) order.push_back(p);
+    double wsum = 0.0;
+    for (int c = 0; c < kLbfgsClasses; ++c) wsum += (double)cls_n[c] * kLbfgsClassVpl[c];
+    int total = 0;
+    for (int c = 0; c < kLbfgsClasses; ++c) {
+      if (!cls_n[c]) continue;
+      int k = (int)std::lround(n_slots * cls_n[c] * kLbfgsClassVpl[c] / wsum);
+      cls_slots[c] = std::max(1, std::min(k, cls_n[c]));
+      total += cls_slots[c];
+    }
+    n_slots = total;
+  }
   if (g_ws.hist_floats < per_slot * n_slots) {
     if (g_ws.hist) HIP_TRY(hipFree(g_ws.hist));
     g_ws.hist = nullptr;
@@ -1331,7 +1371,58 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   a.counter = g_ws.counter;
   a.hist = g_ws.hist;
   a.stats = reinterpret_cast<unsigned long long*>(g_ws.counter + 16);
-  launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
+  if (prm->dof_mode) {
+    launch_lbfgs(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, a, n_slots, s);
+  } else {
+    if (g_ws.order_n < (size_t)n_poses) {
+      if (g_ws.order) HIP_TRY(hipFree(g_ws.order));
+      g_ws.order = nullptr;
+      g_ws.order_n = 0;
+      HIP_TRY(hipMalloc(&g_ws.order, sizeof(int) * (size_t)n_poses));
+      g_ws.order_n = n_poses;
+    }
+    HIP_TRY(hipMemcpyAsync(g_ws.order, order.data(), sizeof(int) * (size_t)n_poses,
+                           hipMemcpyHostToDevice, s));
+    if (!g_ws.fork) {
+      HIP_TRY(hipEventCreateWithFlags(&g_ws.fork, hipEventDisableTiming));
+      for (int k = 0; k < kLbfgsClasses - 1; ++k) {
+        HIP_TRY(hipStreamCreateWithFlags(&g_ws.side[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&g_ws.join[k], hipEventDisableTiming));
+      }
+    }
+    HIP_TRY(hipEventRecord(g_ws.fork, s));
+    // the class with the most work on the caller's stream, the others beside it
+    int main_c = 0;
+    for (int c = 1; c < kLbfgsClasses; ++c)
+      if ((double)cls_n[c] * kLbfgsClassVpl[c] > (double)cls_n[main_c] * kLbfgsClassVpl[main_c])
+        main_c = c;
+    int off = 0, side = 0;
+    bool joined[kLbfgsClasses - 1] = {};
+    size_t hist_off = 0;
+    for (int c = 0; c < kLbfgsClasses; ++c) {
+      LbfgsArgs ac = a;
+      ac.order = g_ws.order + off;
+      ac.n_order = cls_n[c];
+      ac.counter = g_ws.counter + c;
+      ac.hist = g_ws.hist + hist_off;
+      off += cls_n[c];
+      hist_off += per_slot * (size_t)cls_slots[c];
+      if (!cls_n[c]) continue;
+      hipStream_t cs = s;
+      if (c != main_c) {
+        cs = g_ws.side[side];
+        HIP_TRY(hipStreamWaitEvent(cs, g_ws.fork, 0));
+      }
+      launch_lbfgs_class(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, ac, c, cls_slots[c], cs);
+      if (c != main_c) {
+        HIP_TRY(hipEventRecord(g_ws.join[side], cs));
+        joined[side] = true;
+        side++;
+      }
+    }
+    for (int k = 0; k < kLbfgsClasses - 1; ++k)
+      if (joined[k]) HIP_TRY(hipStreamWaitEvent(s, g_ws.join[k], 0));
+  }
   HIP_TRY(hipGetLastError());
   // the shared workspace may be reused by the next call: finish this one
   HIP_TRY(hipStreamSynchronize(s));

@@ -893,14 +893,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
   for (;;) {
     if (lane == 0) s_pose = atomicAdd(A.counter, 1);
     __syncthreads();
-    const int pi = s_pose;
+    const int qi = s_pose;
     __syncthreads();
-    if (pi >= A.n_poses) break;
+    if (qi >= (A.order ? A.n_order : A.n_poses)) break;
+    const int pi = A.order ? A.order[qi] : qi;
     c.li = L.info[A.pose_lig[pi]];
     c.n3 = MODE ? 6 + c.li.n_tors : 3 * c.li.n_atoms;
     c.n = c.n3 + R.n_leaves;
     c.nk = (c.n + kWave - 1) / kWave;
-    if (c.nk <= vpl_lo || c.nk > VPL) continue;  // another size class runs this pose
+    // without a class list every launch scans all poses and takes its own
+    if (!A.order && (c.nk <= vpl_lo || c.nk > VPL)) continue;
     load_ligand_params(L, c.li, G, c.s);
     c.nevals = 0;
     const long long off = 3ll * A.pose_off[pi];
@@ -953,6 +955,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
     }
     __syncthreads();
   }
+}
+
+void launch_lbfgs_class(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                        const LbfgsArgs& a, int cls, int n_slots, hipStream_t st) {
+  LbfgsArgs c = a;
+  c.max_atoms = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  const size_t lds = sizeof(float) * pose_dims(c.max_atoms, R.n_mov, R.n_leaves).total;
+  static_assert(kLbfgsClasses == 3, "one template per class below");
+  if (cls == 2)
+    hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[2], 0>), dim3(n_slots), dim3(kWave), lds, st, R,
+                       L, G, c, 0);
+  else if (cls == 1)
+    hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[1], 0>), dim3(n_slots), dim3(kWave), lds, st, R,
+                       L, G, c, 0);
+  else
+    hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[0], 0>), dim3(n_slots), dim3(kWave), lds, st, R,
+                       L, G, c, 0);
 }
 
 void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,

@@ -63,6 +63,10 @@ struct LbfgsArgs {
   // nullable, zero before launch: totals over the poses of iterations,
   // evaluations, history bytes and map-lookup bytes (pxd_lbfgs_stats)
   unsigned long long* stats;
+  // nullable: the poses of one size class (indices into the pose arrays),
+  // n_order of them; the launch then runs exactly these
+  const int* order;
+  int n_order;
 };
 
 // Per L-BFGS work slot: the y and s histories, [history][64*kLbfgsVpl] each,
@@ -172,6 +176,10 @@ struct ClusterArgs {
 void launch_grid_build(const GridBuildArgs& a, int types_per_pass, hipStream_t st);
 void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const ScoreArgs& a, hipStream_t st);
+// One size class (0, 1, 2: vector width kLbfgsClassVpl[cls]) over the poses
+// a.order lists; the launches of different classes may run concurrently.
+void launch_lbfgs_class(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                        const LbfgsArgs& a, int cls, int n_slots, hipStream_t st);
 void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st);
 void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
