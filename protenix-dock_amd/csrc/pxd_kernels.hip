@@ -1173,16 +1173,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
   for (;;) {
     if (lane == 0) s_w = atomicAdd(A.counter, 1);
     __syncthreads();
-    const int w = s_w;
+    const int qi = s_w;
     __syncthreads();
-    if (w >= A.n_walkers) break;
+    if (qi >= (A.order ? A.n_order : A.n_walkers)) break;
+    const int w = A.order ? A.order[qi] : qi;
     c.li = L.info[A.walker_lig[w]];
     const LigInfo& li = c.li;
     const int n = li.n_atoms, n3 = 3 * n;
     c.n3 = n3;
     c.n = n3 + nl;
     c.nk = (c.n + kWave - 1) / kWave;
-    if (c.nk <= vpl_lo || c.nk > VPL) continue;  // another size class runs this walker
+    // without a class list every launch scans all walkers and takes its own
+    if (!A.order && (c.nk <= vpl_lo || c.nk > VPL)) continue;
     WalkerState st = A.st[w];
     if (st.mode == 2) continue;
     const int pid = A.walker_pid[w];
@@ -1406,6 +1408,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
     if (lane == 0) A.st[w] = st;
     __syncthreads();
   }
+}
+
+void launch_mc_class(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                     const McArgs& a, int cls, int n_slots, hipStream_t st) {
+  McArgs c = a;
+  c.max_atoms = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
+  c.max_frags = a.max_frags > 0 && a.max_frags <= kMaxFrags ? a.max_frags : kMaxFrags;
+  const size_t dyn = sizeof(float) * mc_dims(c.max_atoms, c.max_frags, R.n_mov, R.n_leaves,
+                                             R.n_ring5 + R.n_ring6).total;
+  static_assert(kLbfgsClasses == 3, "one template per class below");
+  if (cls == 2)
+    hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[2]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c, 0);
+  else if (cls == 1)
+    hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[1]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c, 0);
+  else
+    hipLaunchKernelGGL(k_mc<kLbfgsClassVpl[0]>, dim3(n_slots), dim3(kWave), dyn, st, R, L, G, c, 0);
 }
 
 void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,

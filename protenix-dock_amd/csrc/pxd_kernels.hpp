@@ -157,6 +157,10 @@ struct McArgs {
   int step_end;
   int max_atoms;   // largest ligand of the batch (sizes the dynamic LDS)
   int max_frags;   // most fragments of a ligand of the batch
+  // nullable: the walkers of one size class (indices), n_order of them; the
+  // launch then runs exactly these
+  const int* order;
+  int n_order;
 };
 
 struct ClusterArgs {
@@ -186,6 +190,9 @@ void launch_pscore(const DevReceptor& R, const DevLigands& L, const pxd_score_pa
                    const PscoreArgs& a, hipStream_t st);
 void launch_pscore_grad(const DevReceptor& R, const DevLigands& L, const pxd_score_params& sp,
                         const PscoreArgs& a, float* dxyz, double* dparams, hipStream_t st);
+// One size class (0, 1, 2) over the walkers a.order lists.
+void launch_mc_class(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
+                     const McArgs& a, int cls, int n_slots, hipStream_t st);
 void launch_mc(const DevReceptor& R, const DevLigands& L, const DevGrid* G, const McArgs& a,
                int n_slots, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);

@@ -128,6 +128,24 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   const int n_slots = std::min(W, std::max(1, ncu) * 4 * kMcWaves);
   const int hist_m = std::max(prm->relax.history, prm->full.history);
   const int n_launch = (prm->max_nsteps + prm->steps_per_launch - 1) / prm->steps_per_launch + 1;
+  // walkers grouped by L-BFGS size class (their ligand's variable count), in
+  // walker order, so that a class's launch runs its own walkers only; the
+  // class launches follow each other on one stream (a captured graph
+  // replays its kernel nodes in order), each with all the work slots
+  int cls_n[kLbfgsClasses] = {0, 0, 0};
+  std::vector<int> order;
+  order.reserve(W);
+  {
+    std::vector<int8_t> lcls(nl);
+    for (int l = 0; l < nl; ++l) {
+      const int nk = (3 * b->h_natoms[l] + L + kWave - 1) / kWave;
+      lcls[l] = nk <= kLbfgsClassVpl[0] ? 0 : (nk <= kLbfgsClassVpl[1] ? 1 : 2);
+      cls_n[lcls[l]] += walkers;
+    }
+    for (int c = 0; c < kLbfgsClasses; ++c)
+      for (int w = 0; w < W; ++w)
+        if (lcls[wlig[w]] == c) order.push_back(w);
+  }
   const size_t hist_floats = (size_t)n_slots * lbfgs_slot_floats(hist_m);
   hipError_t err = hipSuccess;
   hipStream_t user = static_cast<hipStream_t>(stream), cs = nullptr;
@@ -136,7 +154,7 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   hipGraphExec_t exec = nullptr;
   char* mem = nullptr;
   pxd_score_params* d_sp = nullptr;
-  int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt, *d_coff, *d_cnum;
+  int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt, *d_coff, *d_cnum, *d_order;
   unsigned long long* d_emin;
   float *d_cur, *d_curt, *d_hist;
   WalkerState* d_st;
@@ -151,7 +169,7 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
                o_cnt = take(4 * (size_t)n_launch * kLbfgsClasses), o_cur = take(12 * (size_t)atoms),
                o_curt = take(4 * (size_t)W * (L + 1)), o_st = take(sizeof(WalkerState) * W),
                o_hist = take(4 * hist_floats), o_coff = take(4 * nl), o_cnum = take(4 * nl),
-               o_emin = take(8 * (size_t)nl);
+               o_emin = take(8 * (size_t)nl), o_order = take(4 * (size_t)W);
   HIP_OK(hipMalloc(&mem, off));
   d_sp = reinterpret_cast<pxd_score_params*>(mem + o_sp);
   d_wlig = reinterpret_cast<int*>(mem + o_wlig);
@@ -166,6 +184,7 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   d_coff = reinterpret_cast<int*>(mem + o_coff);
   d_cnum = reinterpret_cast<int*>(mem + o_cnum);
   d_emin = reinterpret_cast<unsigned long long*>(mem + o_emin);
+  d_order = reinterpret_cast<int*>(mem + o_order);
   HIP_OK(hipMemcpy(d_sp, sp, sizeof(*sp), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_wlig, wlig.data(), 4 * W, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_wpid, wpid.data(), 4 * W, hipMemcpyHostToDevice));
@@ -173,9 +192,11 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   HIP_OK(hipMemcpy(d_lwoff, lwoff.data(), 4 * (nl + 1), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_coff, coff.data(), 4 * nl, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_cnum, cnum.data(), 4 * nl, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_order, order.data(), 4 * (size_t)W, hipMemcpyHostToDevice));
   HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+
   HIP_OK(hipEventRecord(ev0, user));
   HIP_OK(hipStreamWaitEvent(cs, ev0, 0));
   {
@@ -223,9 +244,17 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
     // the ligands' lowest pre-optimisation energies start at 1e8 (reader.cpp:51-53)
     launch_fill_u64(d_emin, nl, order_key(1e8), cs);
     for (int k = 0; k < n_launch; ++k) {
-      a.counter = d_cnt + k * kLbfgsClasses;
       a.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
-      launch_mc(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, a, n_slots, cs);
+      for (int c = kLbfgsClasses - 1, off = W; c >= 0; --c) {
+        off -= cls_n[c];
+        if (!cls_n[c]) continue;
+        McArgs ac = a;
+        ac.order = d_order + off;
+        ac.n_order = cls_n[c];
+        ac.counter = d_cnt + k * kLbfgsClasses + c;
+        launch_mc_class(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, ac, c,
+                        std::min(n_slots, cls_n[c]), cs);
+      }
     }
     if (num_modes > 0) {
       ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
@@ -264,6 +293,7 @@ done:
   }
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
+
   if (mem) (void)hipFree(mem);
   if (err != hipSuccess) return pxd_set_error_(PXD_ERR_HIP, hipGetErrorString(err));
   return PXD_OK;
