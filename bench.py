@@ -218,8 +218,10 @@ def search_leg(args, rec, box, R, G, workers):
          "evals_per_s": float(ne.sum() / dt),
          "top9_found": float(res.cluster_count.cpu().numpy().mean()),
          "params": "Gaussian mutation (1 A, 0.3 rad, one torsion), L-BFGS m=8, <=300 "
-                   "iterations, |g|inf <= 1e-3, T = 1.2, top-9 at 0.5 A; steps per chain cut "
-                   "from 2,000 to keep the run short"}
+                   "iterations, |g|inf <= 1e-3, T = 1.2, top-9 at 0.5 A" +
+                   ("" if args.search_steps >= 2000 else
+                    f"; steps per chain cut from 2,000 to {args.search_steps} to keep the run "
+                    "short (--search-steps 2000 runs the full chains)")}
     log(f"search: {d}")
     if not args.no_cpu:
         import pyoracle as O
@@ -289,9 +291,12 @@ def large_leg(args, workers):
     torch.cuda.synchronize()
     grid_s = time.time() - tg
     grid_bytes = (len(types) + 1) * box.n_points() * 16
+    log(f"large: maps built in {grid_s * 1e3:.1f} ms")
     B = lib.LigandBatch(ligs)
+    log(f"large: {len(ligs)} ligands packed")
     pose_lig = np.repeat(np.arange(len(ligs)), per)
     P = lib.PoseSet(B, pose_lig, [p for arr in poses for p in arr], rec.leaves.n_leaves)
+    log(f"large: {P.n_poses} poses resident")
     prm = ffi.lbfgs_params(240)
     torch.cuda.synchronize()
     t1 = time.time()

@@ -1352,7 +1352,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
     int total = 0;
     for (int c = 0; c < kLbfgsClasses; ++c) {
       if (!cls_n[c]) continue;
-      int k = (int)std::lround(n_slots * cls_n[c] * kLbfgsClassVpl[c] / wsum);
+      const int k = (int)std::lround((double)n_slots * cls_n[c] * kLbfgsClassVpl[c] / wsum);
       cls_slots[c] = std::max(1, std::min(k, cls_n[c]));
       total += cls_slots[c];
     }

@@ -534,3 +534,29 @@ def test_gpu_lbfgs_stats_count_the_work(world):
             assert st["history_bytes"] == 0
         else:
             assert st["history_bytes"] > 0 and st["iterations"] > len(ne)
+
+
+def test_gpu_lbfgs_large_batch_keeps_its_work_slots(world):
+    """600,000 poses of one ligand (one size class, 2 L-BFGS iterations each):
+    the work slots shared out among the size classes must not collapse (the
+    share is computed in FP64: slots x poses x width passes 2^31 here), so
+    the batch finishes within seconds, every pose evaluated, and equal
+    starts give equal results."""
+    import time
+    import torch
+    lig = world["ligs"][0]
+    pose_lig, xyz, tors = _poses(world, 1, 97, spread=2.0)
+    n = 600_000
+    B = lib.LigandBatch(world["ligs"])
+    P = lib.PoseSet(B, np.zeros(n, np.int32), [xyz[0]] * n, 8, tors=np.repeat(tors[:1], n, 0))
+    torch.cuda.synchronize()
+    t0 = time.time()
+    e, ne, cv = lib.lbfgs(world["R"], world["G"], P, ffi.lbfgs_params(2))
+    torch.cuda.synchronize()
+    dt = time.time() - t0
+    st = lib.lbfgs_last_stats()
+    e, ne = e.cpu().numpy(), ne.cpu().numpy()
+    print(f"\n{n} poses of {lig.n_atoms} atoms: {dt:.2f} s, {st}")
+    assert dt < 10.0
+    assert st["evaluations"] == int(ne.sum()) and (ne >= 1).all()
+    assert np.all(e == e[0]) and np.all(ne == ne[0])
