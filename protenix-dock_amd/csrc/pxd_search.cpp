@@ -129,10 +129,11 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   const int hist_m = std::max(prm->relax.history, prm->full.history);
   const int n_launch = (prm->max_nsteps + prm->steps_per_launch - 1) / prm->steps_per_launch + 1;
   // walkers grouped by L-BFGS size class (their ligand's variable count), in
-  // walker order, so that a class's launch runs its own walkers only; the
-  // class launches follow each other on one stream (a captured graph
-  // replays its kernel nodes in order), each with all the work slots
-  int cls_n[kLbfgsClasses] = {0, 0, 0};
+  // walker order, so that a class's launches run its own walkers only; each
+  // class's step loop is one captured graph, the graphs replayed side by side
+  // on streams of their own; work slots shared out by each class's walkers
+  // x vector width
+  int cls_n[kLbfgsClasses] = {0, 0, 0}, cls_slots[kLbfgsClasses] = {0, 0, 0};
   std::vector<int> order;
   order.reserve(W);
   {
@@ -145,13 +146,21 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
     for (int c = 0; c < kLbfgsClasses; ++c)
       for (int w = 0; w < W; ++w)
         if (lcls[wlig[w]] == c) order.push_back(w);
+    double wsum = 0.0;
+    for (int c = 0; c < kLbfgsClasses; ++c) wsum += (double)cls_n[c] * kLbfgsClassVpl[c];
+    for (int c = 0; c < kLbfgsClasses; ++c)
+      if (cls_n[c])
+        cls_slots[c] = std::max(1, std::min(cls_n[c], (int)std::lround((double)n_slots * cls_n[c] *
+                                                                       kLbfgsClassVpl[c] / wsum)));
   }
-  const size_t hist_floats = (size_t)n_slots * lbfgs_slot_floats(hist_m);
+  const size_t slot_floats = lbfgs_slot_floats(hist_m);
+  const size_t hist_floats = (size_t)(cls_slots[0] + cls_slots[1] + cls_slots[2]) * slot_floats;
   hipError_t err = hipSuccess;
   hipStream_t user = static_cast<hipStream_t>(stream), cs = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, evj[kLbfgsClasses] = {};
+  hipStream_t side[kLbfgsClasses] = {};
+  hipGraph_t graph[kLbfgsClasses] = {};
+  hipGraphExec_t exec[kLbfgsClasses] = {};
   char* mem = nullptr;
   pxd_score_params* d_sp = nullptr;
   int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt, *d_coff, *d_cnum, *d_order;
@@ -196,6 +205,13 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   HIP_OK(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  HIP_OK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
+  for (int c = 0; c < kLbfgsClasses; ++c)
+    if (cls_n[c]) {
+      HIP_OK(hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking));
+      HIP_OK(hipEventCreateWithFlags(&evj[c], hipEventDisableTiming));
+    }
+
 
   HIP_OK(hipEventRecord(ev0, user));
   HIP_OK(hipStreamWaitEvent(cs, ev0, 0));
@@ -236,36 +252,50 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
     a.hist_m = hist_m;
     a.max_atoms = b->max_atoms;
     a.max_frags = b->max_frags;
-    // capture: reset state and queue heads with fill kernels (no memset nodes),
-    // then one k_mc launch per segment of steps, then clustering
-    HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
-    launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, cs);
-    // the ligands' lowest pre-optimisation energies start at 1e8 (reader.cpp:51-53)
-    launch_fill_u64(d_emin, nl, order_key(1e8), cs);
-    for (int k = 0; k < n_launch; ++k) {
-      a.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
-      for (int c = kLbfgsClasses - 1, off = W; c >= 0; --c) {
-        off -= cls_n[c];
-        if (!cls_n[c]) continue;
-        McArgs ac = a;
-        ac.order = d_order + off;
-        ac.n_order = cls_n[c];
+    // each class's step loop captured once: one k_mc launch per segment of
+    // steps over the class's walkers, with its own queue heads, work slots
+    // and history scratch
+    size_t hist_off = 0;
+    for (int c = 0, off = 0; c < kLbfgsClasses; ++c) {
+      McArgs ac = a;
+      ac.order = d_order + off;
+      ac.n_order = cls_n[c];
+      ac.hist = d_hist + hist_off;
+      off += cls_n[c];
+      hist_off += slot_floats * (size_t)cls_slots[c];
+      if (!cls_n[c]) continue;
+      HIP_OK(hipStreamBeginCapture(side[c], hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < n_launch; ++k) {
+        ac.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
         ac.counter = d_cnt + k * kLbfgsClasses + c;
         launch_mc_class(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, ac, c,
-                        std::min(n_slots, cls_n[c]), cs);
+                        cls_slots[c], side[c]);
       }
+      err = hipStreamEndCapture(side[c], &graph[c]);
+      if (err != hipSuccess) goto done;
+      HIP_OK(hipGraphInstantiate(&exec[c], graph[c], nullptr, nullptr, 0));
     }
-    if (num_modes > 0) {
-      ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
-                     cluster_idx, cluster_count};
-      launch_cluster(ca, 0, cs);
-    }
-    err = hipStreamEndCapture(cs, &graph);
-    if (err != hipSuccess) goto done;
   }
-  HIP_OK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  HIP_OK(hipGraphLaunch(exec, cs));
+  // reset state and queue heads with fill kernels (no memset), then the
+  // class graphs side by side, then the clustering
+  launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
+  launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, cs);
+  // the ligands' lowest pre-optimisation energies start at 1e8 (reader.cpp:51-53)
+  launch_fill_u64(d_emin, nl, order_key(1e8), cs);
+  HIP_OK(hipEventRecord(evf, cs));
+  for (int c = 0; c < kLbfgsClasses; ++c) {
+    if (!cls_n[c]) continue;
+    HIP_OK(hipStreamWaitEvent(side[c], evf, 0));
+    HIP_OK(hipGraphLaunch(exec[c], side[c]));
+    HIP_OK(hipEventRecord(evj[c], side[c]));
+    HIP_OK(hipStreamWaitEvent(cs, evj[c], 0));
+  }
+  if (num_modes > 0) {
+    ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
+                   cluster_idx, cluster_count};
+    launch_cluster(ca, 0, cs);
+  }
+  HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(cs));
   {
     // per-walker scalars out of the walker state
@@ -285,8 +315,16 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   HIP_OK(hipEventRecord(ev1, cs));
   HIP_OK(hipStreamWaitEvent(user, ev1, 0));
 done:
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  for (int c = 0; c < kLbfgsClasses; ++c) {
+    if (exec[c]) (void)hipGraphExecDestroy(exec[c]);
+    if (graph[c]) (void)hipGraphDestroy(graph[c]);
+    if (side[c]) {
+      (void)hipStreamSynchronize(side[c]);
+      (void)hipStreamDestroy(side[c]);
+    }
+    if (evj[c]) (void)hipEventDestroy(evj[c]);
+  }
+  if (evf) (void)hipEventDestroy(evf);
   if (cs) {
     (void)hipStreamSynchronize(cs);
     (void)hipStreamDestroy(cs);
