@@ -715,31 +715,36 @@ def main():
             # energy (1e-3 kcal/mol) from the same start
             from concurrent.futures import ThreadPoolExecutor
 
-            def noisy_one(i):
+            def noisy_one(i, level):
                 p = pidx[i]
-                O.lib().or_set_eval_noise(1e-7, 1000003 + i)
+                O.lib().or_set_eval_noise(level, 1000003 + i)
                 try:
                     return systems[pose_lig[p]].lbfgs(
                         flat[offs[p]:offs[p] + ligs[pose_lig[p]].n_atoms],
                         np.zeros(rec.leaves.n_leaves), prm)["energy"]
                 finally:
                     O.lib().or_set_eval_noise(0.0, 0)
+            # 1e-7: far below FP32; 1e-5: the FP32 evaluation's own error scale
             with ThreadPoolExecutor(threads) as ex:
-                noisy = list(ex.map(noisy_one, range(nc)))
-            m = len(noisy)
+                noisy7 = list(ex.map(lambda i: noisy_one(i, 1e-7), range(nc)))
+                noisy5 = list(ex.map(lambda i: noisy_one(i, 1e-5), range(nc)))
+            m = len(noisy7)
             dev = le[cs]
             lb["parity_vs_fp64_oracle"] = {
                 "poses": nc,
                 "same_final_energy_frac": float(np.mean(np.abs(dev - ce) <= 1e-3)),
                 "oracle_self_same_final_energy_frac_under_1e-7_noise":
-                    float(np.mean(np.abs(np.array(noisy) - ce[:m]) <= 1e-3)),
+                    float(np.mean(np.abs(np.array(noisy7) - ce[:m]) <= 1e-3)),
+                "oracle_self_same_final_energy_frac_under_1e-5_noise":
+                    float(np.mean(np.abs(np.array(noisy5) - ce[:m]) <= 1e-3)),
                 "oracle_self_poses": m,
                 "final_energy_ks_pvalue": float(stats.ks_2samp(dev, ce).pvalue),
                 "converged_frac_device": float(lcv[cs].mean()),
                 "converged_frac_oracle": float(ccv.mean()),
                 "note": "same-start final energies of the reference L-BFGS are not reproducible "
-                        "under a 1e-7 perturbation of its evaluations; parity is the "
-                        "distribution (KS), the converged fraction, and the trajectory prefix "
+                        "under a 1e-7 perturbation of its evaluations (nor under 1e-5, the "
+                        "FP32 evaluation error); parity is the distribution (KS), the "
+                        "converged fraction, and the trajectory prefix "
                         "(tests/test_gpu_lbfgs_parity.py)"}
             log(f"L-BFGS cpu {lb['cpu']}; parity {lb['parity_vs_fp64_oracle']}")
     # ---------------- docking search: device MC walkers (bounded sample of the
