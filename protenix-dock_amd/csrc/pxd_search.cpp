@@ -161,6 +161,12 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   hipStream_t side[kLbfgsClasses] = {};
   hipGraph_t graph[kLbfgsClasses] = {};
   hipGraphExec_t exec[kLbfgsClasses] = {};
+  // the class with the most work replays on cs, the others on side streams
+  // (fewer streams than the process's hardware queues)
+  int main_c = 0;
+  for (int c = 1; c < kLbfgsClasses; ++c)
+    if ((double)cls_n[c] * kLbfgsClassVpl[c] > (double)cls_n[main_c] * kLbfgsClassVpl[main_c])
+      main_c = c;
   char* mem = nullptr;
   pxd_score_params* d_sp = nullptr;
   int *d_wlig, *d_wpid, *d_woff, *d_lwoff, *d_cnt, *d_coff, *d_cnum, *d_order;
@@ -207,7 +213,7 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
   HIP_OK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
   for (int c = 0; c < kLbfgsClasses; ++c)
-    if (cls_n[c]) {
+    if (cls_n[c] && c != main_c) {
       HIP_OK(hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking));
       HIP_OK(hipEventCreateWithFlags(&evj[c], hipEventDisableTiming));
     }
@@ -264,14 +270,15 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
       off += cls_n[c];
       hist_off += slot_floats * (size_t)cls_slots[c];
       if (!cls_n[c]) continue;
-      HIP_OK(hipStreamBeginCapture(side[c], hipStreamCaptureModeThreadLocal));
+      hipStream_t cap = c == main_c ? cs : side[c];
+      HIP_OK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < n_launch; ++k) {
         ac.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
         ac.counter = d_cnt + k * kLbfgsClasses + c;
         launch_mc_class(r->dev, b->dev, g ? static_cast<const DevGrid*>(g->mem) : nullptr, ac, c,
-                        cls_slots[c], side[c]);
+                        cls_slots[c], cap);
       }
-      err = hipStreamEndCapture(side[c], &graph[c]);
+      err = hipStreamEndCapture(cap, &graph[c]);
       if (err != hipSuccess) goto done;
       HIP_OK(hipGraphInstantiate(&exec[c], graph[c], nullptr, nullptr, 0));
     }
@@ -284,12 +291,14 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   launch_fill_u64(d_emin, nl, order_key(1e8), cs);
   HIP_OK(hipEventRecord(evf, cs));
   for (int c = 0; c < kLbfgsClasses; ++c) {
-    if (!cls_n[c]) continue;
+    if (!cls_n[c] || c == main_c) continue;
     HIP_OK(hipStreamWaitEvent(side[c], evf, 0));
     HIP_OK(hipGraphLaunch(exec[c], side[c]));
     HIP_OK(hipEventRecord(evj[c], side[c]));
-    HIP_OK(hipStreamWaitEvent(cs, evj[c], 0));
   }
+  if (cls_n[main_c]) HIP_OK(hipGraphLaunch(exec[main_c], cs));
+  for (int c = 0; c < kLbfgsClasses; ++c)
+    if (cls_n[c] && c != main_c) HIP_OK(hipStreamWaitEvent(cs, evj[c], 0));
   if (num_modes > 0) {
     ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
                    cluster_idx, cluster_count};
