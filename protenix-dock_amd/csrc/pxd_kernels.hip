@@ -352,8 +352,11 @@ void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
 // s/y history lives in the slot's global scratch; scalar control (f, t, gtd,
 // curvature s.y / y.y, Armijo) is FP64.
 
-// Per-wave LDS vector pool.
-enum { V_X = 0, V_G, V_GP, V_D, V_W0, V_W1, V_W2, V_W3, V_COUNT };
+// Per-wave LDS vector pool: the variables, gradient, previous gradient,
+// direction and three line-search buffers (the bracketing phase holds the
+// new and the previous point's gradients, the zoom the bracket's two ends
+// and the new point's: never more than three at once).
+enum { V_X = 0, V_G, V_GP, V_D, V_W0, V_W1, V_W2, V_COUNT };
 
 template <int VPL>
 struct LbfgsCtx {
@@ -536,7 +539,9 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
   double t = *t_io, f = *f_io;
   int b_new = V_W0, b_prev = V_W1;
   Pair2 br{0, 0}, bf{0, 0}, bgtd{0, 0};
-  IPair2 idx{V_W2, V_W3};  // buffers of bracket_g[0], bracket_g[1]
+  // buffers of bracket_g[0], bracket_g[1]: always assigned below before the
+  // zoom, from the two bracketing buffers V_W0 / V_W1
+  IPair2 idx{V_W0, V_W1};
   double f_new = eval_dir(c, t, d, c.vec(b_new));
   if (isnan(f_new)) {
     *f_io = f_new;
