@@ -212,8 +212,7 @@ struct LbfgsWorkspace {
   pxd_lbfgs_stats last{};  // totals of the last pxd_lbfgs_batch
   int* order = nullptr;    // poses grouped by size class
   size_t order_n = 0;
-  // side streams (default priority) and events of the concurrent class launches
-  hipStream_t side[kLbfgsClasses - 1] = {};
+  // events of the class launches side by side (on side_streams())
   hipEvent_t fork = nullptr, join[kLbfgsClasses - 1] = {};
 };
 LbfgsWorkspace g_ws;
@@ -1231,6 +1230,35 @@ static pxd_status check_grid_types(const pxd_grid* g, const pxd_ligand_batch* b)
 
 namespace pxd {
 
+namespace {
+constexpr int kMaxSideDevices = 64;
+std::mutex g_side_mu;
+hipStream_t g_side[kMaxSideDevices][kSideStreams] = {};
+}  // namespace
+
+static_assert(kLbfgsClasses - 1 <= kSideStreams,
+              "every size class but one needs a side stream");
+
+std::mutex& side_stream_mutex() { return g_side_mu; }
+
+hipError_t side_streams(hipStream_t out[kSideStreams]) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxSideDevices) return hipErrorInvalidDevice;
+  for (int k = 0; k < kSideStreams; ++k) {
+    if (!g_side[dev][k]) {
+      e = hipStreamCreateWithFlags(&g_side[dev][k], hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        g_side[dev][k] = nullptr;
+        return e;
+      }
+    }
+    out[k] = g_side[dev][k];
+  }
+  return hipSuccess;
+}
+
 // Argument checks of pxd_lbfgs_batch and its launch arguments (all but the
 // queue heads and the history storage), shared with the entry point that
 // allocates nothing.  *n_slots: the work slots the tuning asks for (at most
@@ -1325,6 +1353,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
   hipStream_t s = static_cast<hipStream_t>(stream);
   size_t per_slot = lbfgs_slot_floats(prm->history);
   std::lock_guard<std::mutex> lock(g_ws.mu);
+  std::lock_guard<std::mutex> side_lock(side_stream_mutex());
   // Cartesian variables: the poses grouped by size class on the host (stable
   // order), one launch per class, the classes side by side on streams of
   // their own so that one class's tail overlaps the others' work; work
@@ -1385,11 +1414,11 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
                            hipMemcpyHostToDevice, s));
     if (!g_ws.fork) {
       HIP_TRY(hipEventCreateWithFlags(&g_ws.fork, hipEventDisableTiming));
-      for (int k = 0; k < kLbfgsClasses - 1; ++k) {
-        HIP_TRY(hipStreamCreateWithFlags(&g_ws.side[k], hipStreamNonBlocking));
+      for (int k = 0; k < kLbfgsClasses - 1; ++k)
         HIP_TRY(hipEventCreateWithFlags(&g_ws.join[k], hipEventDisableTiming));
-      }
     }
+    hipStream_t side_s[kSideStreams];
+    HIP_TRY(side_streams(side_s));
     HIP_TRY(hipEventRecord(g_ws.fork, s));
     // the class with the most work on the caller's stream, the others beside it
     int main_c = 0;
@@ -1410,7 +1439,7 @@ pxd_status pxd_lbfgs_batch(const pxd_receptor* r, const pxd_grid* g, const pxd_l
       if (!cls_n[c]) continue;
       hipStream_t cs = s;
       if (c != main_c) {
-        cs = g_ws.side[side];
+        cs = side_s[side];
         HIP_TRY(hipStreamWaitEvent(cs, g_ws.fork, 0));
       }
       launch_lbfgs_class(r->dev, b->dev, g ? grid_dev_ptr(g) : nullptr, ac, c, cls_slots[c], cs);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <vector>
 
 #include "pxdock.h"
@@ -180,6 +181,17 @@ struct DevGrid {
   // (value, dE/dx, dE/dy, dE/dz)
   const float* smaps;
 };
+
+// The two default-priority side streams of the current device on which the
+// batched L-BFGS and the search run their smaller size classes beside the
+// caller's stream.  They are created once per device and kept: a process has
+// only a few hardware queues (GPU_MAX_HW_QUEUES, 4 by default), and streams
+// beyond them share a queue with another stream, which serialises the class
+// launches that were meant to overlap.  A caller holds side_stream_mutex()
+// from its first use of the streams until its work on them has finished.
+constexpr int kSideStreams = 2;
+std::mutex& side_stream_mutex();
+hipError_t side_streams(hipStream_t out[kSideStreams]);
 
 }  // namespace pxd
 

@@ -156,13 +156,14 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   const size_t slot_floats = lbfgs_slot_floats(hist_m);
   const size_t hist_floats = (size_t)(cls_slots[0] + cls_slots[1] + cls_slots[2]) * slot_floats;
   hipError_t err = hipSuccess;
-  hipStream_t user = static_cast<hipStream_t>(stream), cs = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evf = nullptr, evj[kLbfgsClasses] = {};
-  hipStream_t side[kLbfgsClasses] = {};
+  hipStream_t user = static_cast<hipStream_t>(stream);
+  hipEvent_t evf = nullptr, evj[kLbfgsClasses] = {};
+  hipStream_t side[kLbfgsClasses] = {}, pool[kSideStreams] = {};
   hipGraph_t graph[kLbfgsClasses] = {};
   hipGraphExec_t exec[kLbfgsClasses] = {};
-  // the class with the most work replays on cs, the others on side streams
-  // (fewer streams than the process's hardware queues)
+  // the class with the most work replays on the caller's stream, the others
+  // on the device's two shared side streams (no more streams than the
+  // process's hardware queues)
   int main_c = 0;
   for (int c = 1; c < kLbfgsClasses; ++c)
     if ((double)cls_n[c] * kLbfgsClassVpl[c] > (double)cls_n[main_c] * kLbfgsClassVpl[main_c])
@@ -185,6 +186,7 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
                o_curt = take(4 * (size_t)W * (L + 1)), o_st = take(sizeof(WalkerState) * W),
                o_hist = take(4 * hist_floats), o_coff = take(4 * nl), o_cnum = take(4 * nl),
                o_emin = take(8 * (size_t)nl), o_order = take(4 * (size_t)W);
+  std::unique_lock<std::mutex> side_lock(side_stream_mutex());
   HIP_OK(hipMalloc(&mem, off));
   d_sp = reinterpret_cast<pxd_score_params*>(mem + o_sp);
   d_wlig = reinterpret_cast<int*>(mem + o_wlig);
@@ -208,19 +210,13 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   HIP_OK(hipMemcpy(d_coff, coff.data(), 4 * nl, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_cnum, cnum.data(), 4 * nl, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_order, order.data(), 4 * (size_t)W, hipMemcpyHostToDevice));
-  HIP_OK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  HIP_OK(hipEventCreateWithFlags(&ev0, hipEventDisableTiming));
-  HIP_OK(hipEventCreateWithFlags(&ev1, hipEventDisableTiming));
+  HIP_OK(side_streams(pool));
   HIP_OK(hipEventCreateWithFlags(&evf, hipEventDisableTiming));
-  for (int c = 0; c < kLbfgsClasses; ++c)
+  for (int c = 0, k = 0; c < kLbfgsClasses; ++c)
     if (cls_n[c] && c != main_c) {
-      HIP_OK(hipStreamCreateWithFlags(&side[c], hipStreamNonBlocking));
+      side[c] = pool[k++];
       HIP_OK(hipEventCreateWithFlags(&evj[c], hipEventDisableTiming));
     }
-
-
-  HIP_OK(hipEventRecord(ev0, user));
-  HIP_OK(hipStreamWaitEvent(cs, ev0, 0));
   {
     McArgs a;
     memset(&a, 0, sizeof(a));
@@ -270,7 +266,9 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
       off += cls_n[c];
       hist_off += slot_floats * (size_t)cls_slots[c];
       if (!cls_n[c]) continue;
-      hipStream_t cap = c == main_c ? cs : side[c];
+      // captured on a shared side stream (the caller's may be the null
+      // stream, which cannot capture); replayed below
+      hipStream_t cap = pool[0];
       HIP_OK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < n_launch; ++k) {
         ac.step_end = std::min(prm->max_nsteps, k * prm->steps_per_launch);
@@ -285,27 +283,27 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
   }
   // reset state and queue heads with fill kernels (no memset), then the
   // class graphs side by side, then the clustering
-  launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, cs);
-  launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, cs);
+  launch_fill_int(reinterpret_cast<int*>(d_st), (int)(sizeof(WalkerState) / 4 * W), 0, user);
+  launch_fill_int(d_cnt, n_launch * kLbfgsClasses, 0, user);
   // the ligands' lowest pre-optimisation energies start at 1e8 (reader.cpp:51-53)
-  launch_fill_u64(d_emin, nl, order_key(1e8), cs);
-  HIP_OK(hipEventRecord(evf, cs));
+  launch_fill_u64(d_emin, nl, order_key(1e8), user);
+  HIP_OK(hipEventRecord(evf, user));
   for (int c = 0; c < kLbfgsClasses; ++c) {
     if (!cls_n[c] || c == main_c) continue;
     HIP_OK(hipStreamWaitEvent(side[c], evf, 0));
     HIP_OK(hipGraphLaunch(exec[c], side[c]));
     HIP_OK(hipEventRecord(evj[c], side[c]));
   }
-  if (cls_n[main_c]) HIP_OK(hipGraphLaunch(exec[main_c], cs));
+  if (cls_n[main_c]) HIP_OK(hipGraphLaunch(exec[main_c], user));
   for (int c = 0; c < kLbfgsClasses; ++c)
-    if (cls_n[c] && c != main_c) HIP_OK(hipStreamWaitEvent(cs, evj[c], 0));
+    if (cls_n[c] && c != main_c) HIP_OK(hipStreamWaitEvent(user, evj[c], 0));
   if (num_modes > 0) {
     ClusterArgs ca{nl, d_lwoff, d_woff, d_wlig, best_xyz, d_st, num_modes, (float)min_rmsd,
                    cluster_idx, cluster_count};
-    launch_cluster(ca, 0, cs);
+    launch_cluster(ca, 0, user);
   }
   HIP_OK(hipGetLastError());
-  HIP_OK(hipStreamSynchronize(cs));
+  HIP_OK(hipStreamSynchronize(user));
   {
     // per-walker scalars out of the walker state
     std::vector<WalkerState> st(W);
@@ -321,25 +319,16 @@ pxd_status mc_search(const pxd_receptor* r, const pxd_grid* g, const pxd_ligand_
     HIP_OK(hipMemcpy(best_pscore, bp.data(), 8 * W, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(nevals, ne.data(), 4 * W, hipMemcpyHostToDevice));
   }
-  HIP_OK(hipEventRecord(ev1, cs));
-  HIP_OK(hipStreamWaitEvent(user, ev1, 0));
 done:
   for (int c = 0; c < kLbfgsClasses; ++c) {
     if (exec[c]) (void)hipGraphExecDestroy(exec[c]);
     if (graph[c]) (void)hipGraphDestroy(graph[c]);
-    if (side[c]) {
-      (void)hipStreamSynchronize(side[c]);
-      (void)hipStreamDestroy(side[c]);
-    }
+    // the side streams are kept; their work must finish before the memory goes
+    if (side[c]) (void)hipStreamSynchronize(side[c]);
     if (evj[c]) (void)hipEventDestroy(evj[c]);
   }
   if (evf) (void)hipEventDestroy(evf);
-  if (cs) {
-    (void)hipStreamSynchronize(cs);
-    (void)hipStreamDestroy(cs);
-  }
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
+  (void)hipStreamSynchronize(user);
 
   if (mem) (void)hipFree(mem);
   if (err != hipSuccess) return pxd_set_error_(PXD_ERR_HIP, hipGetErrorString(err));
