@@ -71,6 +71,18 @@ __device__ inline double wave_sum(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Sums over each half-wave: *lo of lanes 0-31, *hi of lanes 32-63 (both
+// wave-uniform).
+__device__ inline void half_sums(float v, float* lo, float* hi) {
+  v += dpp_f<0xB1, 0xF>(v);
+  v += dpp_f<0x4E, 0xF>(v);
+  v += dpp_f<0x141, 0xF>(v);
+  v += dpp_f<0x140, 0xF>(v);
+  v += dpp_f<0x142, 0xA>(v);
+  *lo = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  *hi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 template <int CTRL, int ROWS>
 __device__ inline float dpp_fmax_src(float v) {  // disabled rows read -inf
   return __int_as_float(__builtin_amdgcn_update_dpp((int)0xff800000u, __float_as_int(v), CTRL,
@@ -746,8 +758,15 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
 __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& li,
                                            const PoseLds& s, float* g0) {
   const int lane = lane_id(), n = li.n_atoms;
-  const bool act = lane < n;
-  const int a = act ? lane : 0;
+  // Ligands of at most 32 atoms: both half-waves own the atoms (lane mod 32)
+  // and split the ring steps, the lower half taking steps 1 .. K0, the upper
+  // K0 + 1 .. K; each half runs its own ring of travelling accumulators and
+  // the halves' sums are added at the end.
+  const bool split = n <= kWave / 2;
+  const int half = split ? lane >> 5 : 0, base = half << 5;
+  const int al = split ? (lane & 31) : lane;
+  const bool act = al < n;
+  const int a = act ? al : 0;
   const uint4* pm = L.pmask + 2ll * li.atom_off;
   uint64_t ra = 0, rf = 0;
   if (act) {
@@ -762,7 +781,7 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   const float2 l0 = s.lj[a];
   static_assert(2 * kWave * 8 <= 4 * kStageSlots, "2n records of 8 floats fit the gradient region");
   float4* rec = reinterpret_cast<float4*>(s.grad);  // 2n records of 2 float4 (n <= 64)
-  if (act) {
+  if (act && !half) {
     const float4 r0 = make_float4(x0, y0, z0, q0), r1 = make_float4(0.5f * l0.x, l0.y, 0.f, 0.f);
     rec[2 * a] = r0;
     rec[2 * a + 1] = r1;
@@ -772,19 +791,28 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   wave_sync();
   // own factors of the pair parameters: q_a * kCoulomb, sigma_a / 2, 4 sqrt(eps_a)
   const float q0k = q0 * kCoulomb, sh0 = 0.5f * l0.x, e40 = 4.f * l0.y;
-  const int nxt = act ? (a + 1 == n ? 0 : a + 1) : lane;  // ring successor (source of t)
+  // ring successor (source of t) within the half
+  const int nxt = act ? base + (a + 1 == n ? 0 : a + 1) : lane;
   const int K = n / 2;
-  float t[3] = {0.f, 0.f, 0.f}, es = 0.f;
-  const float4* pr = rec + 2 * (a + 1);  // partner a + k's record in step k
+  const int K0 = split ? (K + 1) / 2 : K;  // steps per half: k0 + 1 .. k0 + K0
+  const int k0 = half * K0;
+  float gr[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f}, es = 0.f;
+  const float4* pr = rec + 2 * (a + 1 + k0);  // partner a + k's record in step k
   // bits 1 .. 32 of the rotated masks (K <= 32 steps), bit k - 1 = step k;
   // with n even, the pair (a, a + n/2) is met from a < n/2 only: lanes
-  // a >= n/2 drop step n/2 (inactive lanes have empty masks)
+  // a >= n/2 drop step n/2 (inactive lanes have empty masks).  The upper
+  // half shifts its steps to bit 0 and drops those past K.
   uint32_t ba = (uint32_t)(ra >> 1), bf = (uint32_t)(rf >> 1);
   if (K > 0 && 2 * K == n && a >= K) {
     ba &= ~(1u << (K - 1));
     bf &= ~(1u << (K - 1));
   }
-  for (int k = 1; k <= K; ++k, pr += 2) {
+  if (half) {
+    const uint32_t keep = (1u << (K - k0)) - 1;  // K - k0 <= 8 here
+    ba = (ba >> k0) & keep;
+    bf = (bf >> k0) & keep;
+  }
+  for (int k = 1; k <= K0; ++k, pr += 2) {
     const uint32_t ia = __builtin_amdgcn_ubfe(ba, k - 1, 1), i4 = __builtin_amdgcn_ubfe(bf, k - 1, 1);
     if (ia | i4) {
       const float fa = (float)ia, f4 = (float)i4;  // scales: 1, the 1-4 scale, or their sum
@@ -794,26 +822,38 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
       const float rx = pb.x - x0, ry = pb.y - y0, rz = pb.z - z0;
       float de;
       es += pair_energy_qk(rx, ry, rz, q0k * pb.w * cs, sh0 + lb.x, e40 * lb.y * vs, &de);
-      g0[0] -= rx * de;
-      g0[1] -= ry * de;
-      g0[2] -= rz * de;
+      gr[0] -= rx * de;
+      gr[1] -= ry * de;
+      gr[2] -= rz * de;
       t[0] += rx * de;
       t[1] += ry * de;
       t[2] += rz * de;
     }
-    if (k < K) {
+    if (k < K0) {
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         t[d] = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * nxt, __float_as_int(t[d])));
     }
   }
-  // t of lane a belongs to atom a + K: atom j's sum sits in lane j - K
-  if (K > 0) {
-    int home = act ? a - K : lane;
+  // t of lane a belongs to atom a + k0 + K0: atom j's sum sits in lane
+  // j - k0 - K0 of the same half
+  if (K0 > 0) {
+    int home = a - k0 - K0;
     if (home < 0) home += n;
+    home = act ? base + home : lane;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      g0[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * home, __float_as_int(t[d])));
+      gr[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * home, __float_as_int(t[d])));
+  }
+  if (split) {  // the upper half's sums to the owning lane of the lower half
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      gr[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * (lane ^ 32), __float_as_int(gr[d])));
+  }
+  if (!half) {
+    g0[0] += gr[0];
+    g0[1] += gr[1];
+    g0[2] += gr[2];
   }
   return (double)es;
 }
@@ -1109,6 +1149,60 @@ __device__ inline double inter(const DevReceptor& R, const DevGrid* G, const Lig
   // sigma / 2, 4 sqrt(eps)) are formed once per atom for the wave; the pair
   // energies are summed in FP32 per lane, then added to the FP64 total.
   float tl = 0.f, em = 0.f;
+  if (n <= kWave / 2) {
+    // Ligands of at most 32 atoms: both half-waves own the atoms (lane mod
+    // 32) and take alternate movable slots, the lower half slot 2i and the
+    // upper 2i + 1; one pass of half-wave sums gives both slots' jac . g.  A
+    // half's running sum is flushed whenever the next slot of either half
+    // starts another leaf (a partial flush adds to the same leaf later).
+    const int half = lane >> 5, am = lane & 31, M = R.n_mov;
+    const bool act = am < n;
+    const int ac = act ? am : 0;
+    const float xa = s.xyz[3 * ac], ya = s.xyz[3 * ac + 1], za = s.xyz[3 * ac + 2];
+    const float2 la = s.lj[ac];
+    const float qa = s.q[ac], sa = 0.5f * la.x;
+    float gm[3] = {0.f, 0.f, 0.f};
+    for (int i = 0; 2 * i < M; ++i) {
+      const int mlo = 2 * i, mhi = 2 * i + 1;
+      const bool has_hi = mhi < M;
+      const int m = half && has_hi ? mhi : mlo;
+      float gx = 0.f, gy = 0.f, gz = 0.f;
+      if (act && (!half || has_hi)) {
+        const float4 mp = s.movp[m];
+        const float rx = s.mov[3 * m] - xa, ry = s.mov[3 * m + 1] - ya, rz = s.mov[3 * m + 2] - za;
+        float de;
+        em += pair_energy_qk(rx, ry, rz, qa * (mp.x * kCoulomb), fmaf(mp.y, 0.5f, sa),
+                             la.y * (4.f * mp.z), &de);
+        gx = rx * de;
+        gy = ry * de;
+        gz = rz * de;
+        gm[0] -= gx;
+        gm[1] -= gy;
+        gm[2] -= gz;
+      }
+      tl += s.jac[3 * m] * gx + s.jac[3 * m + 1] * gy + s.jac[3 * m + 2] * gz;
+      const int leaf_lo = R.mov_leaf[mlo], leaf_hi = has_hi ? R.mov_leaf[mhi] : -1;
+      const bool flush = mlo + 2 >= M || R.mov_leaf[mlo + 2] != leaf_lo ||
+                         (has_hi && (mhi + 2 >= M || R.mov_leaf[mhi + 2] != leaf_hi));
+      if (flush) {
+        float tlo, thi;
+        half_sums(tl, &tlo, &thi);
+        if (lane == 0) {
+          s.torsg[leaf_lo] += tlo;
+          if (has_hi) s.torsg[leaf_hi] += thi;
+        }
+        tl = 0.f;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      gm[d] += __int_as_float(__builtin_amdgcn_ds_bpermute(4 * (lane ^ 32), __float_as_int(gm[d])));
+    if (!half) {
+      g0[0] += gm[0];
+      g0[1] += gm[1];
+      g0[2] += gm[2];
+    }
+  } else
   for (int m = 0; m < R.n_mov; ++m) {
     const float4 mp = s.movp[m];
     const float qmk = mp.x * kCoulomb, smh = 0.5f * mp.y, em4 = 4.f * mp.z;
