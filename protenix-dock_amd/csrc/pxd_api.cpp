@@ -395,6 +395,7 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   std::vector<std::vector<float4>> seg_pos(M);
   std::vector<std::vector<float2>> seg_lj(M);
   std::vector<int4> mm;
+  std::vector<float4> mm_par;
   for (int pass = 0; pass < 2; ++pass) {
     int cnt = pass == 0 ? m->n_pairs14 : m->n_pairs;
     const int32_t* ij = pass == 0 ? m->pairs14 : m->pairs;
@@ -407,7 +408,11 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
         double e4 = std::sqrt(m->epsilon[i] * m->epsilon[j]) * vs * 4.0;
         e_const += h_pair(m->charge[i] * m->charge[j] * cs, sig, e4, h_dist(xyz + 3 * i, xyz + 3 * j));
       } else if (si >= 0 && sj >= 0) {
-        mm.push_back(make_int4(si, sj, pass == 0 ? 1 : 0, 0));
+        mm.push_back(make_int4(si, sj, pass == 0 ? 1 : 0, mov_leaf[si] | mov_leaf[sj] << 16));
+        mm_par.push_back(make_float4((float)(m->charge[i] * m->charge[j] * cs * 332.0637141491396),
+                                     (float)((m->sigma[i] + m->sigma[j]) * 0.5),
+                                     (float)(std::sqrt(m->epsilon[i] * m->epsilon[j]) * 4.0 * vs),
+                                     0.f));
       } else {
         int mv = si >= 0 ? si : sj, fx = si >= 0 ? j : i;
         seg_pos[mv].push_back(make_float4((float)xyz[3 * fx], (float)xyz[3 * fx + 1],
@@ -496,6 +501,7 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
     }
     auto idx = by_class(cls, mm_seg);
     permute(mm, idx, 1);
+    permute(mm_par, idx, 1);
   }
   for (int sl = 0; sl < M; ++sl) leaf_slot[mov_leaf[sl] + 1] = sl + 1;
   for (int k = 0; k < nl; ++k) leaf_slot[k + 1] = std::max(leaf_slot[k + 1], leaf_slot[k]);
@@ -529,7 +535,7 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
          o_movleaf = P.add(mov_leaf), o_movatom = P.add(mov_atom), o_bab = P.add(bab),
          o_bkr = P.add(bkr), o_aabc = P.add(aabc), o_akt = P.add(akt), o_dabcd = P.add(dabcd),
          o_dper = P.add(dper), o_dkp = P.add(dkp), o_rpseg = P.add(rp_seg),
-         o_rppos = P.add(rp_pos), o_rplj = P.add(rp_lj), o_mm = P.add(mm),
+         o_rppos = P.add(rp_pos), o_rplj = P.add(rp_lj), o_mm = P.add(mm), o_mmpar = P.add(mm_par),
          o_hydro = P.add(hydro), o_cat = P.add(cat), o_an = P.add(an), o_r5 = P.add(r5),
          o_r6 = P.add(r6), o_donc = P.add(donc), o_donn = P.add(donn), o_accc = P.add(accc),
          o_accn = P.add(accn), o_slot = P.add(slot_of), o_bseg = P.add(bond_seg),
@@ -576,6 +582,7 @@ pxd_status pxd_receptor_create(const pxd_mol* m, const double* xyz, const pxd_le
   D.rp_lj = dptr<float2>(B, o_rplj);
   D.n_mm = (int)mm.size();
   D.mm = dptr<int4>(B, o_mm);
+  D.mm_par = dptr<float4>(B, o_mmpar);
   D.n_hydro = (int)hydro.size();
   D.n_cation = (int)cat.size();
   D.n_anion = (int)an.size();

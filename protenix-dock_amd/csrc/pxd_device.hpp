@@ -522,19 +522,22 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
     rec_grad_add(s, id.z, g[2], -dedt);
     rec_grad_add(s, id.w, g[3], -dedt);
   }
-  // movable-vs-movable pairs
+  // movable-vs-movable pairs (parameters formed on the host).  A slot's
+  // gradient only enters through its leaf's dE/dtheta = jac . g, so each
+  // pair adds jac . g of its two slots to the two leaves' torsion gradients
+  // directly (two LDS atomics instead of six on the slot gradients).
   for (int k = m0 + lane; k < R.n_mm; k += kWave) {
-    int4 mm = R.mm[k];
-    float cs = mm.z ? kScale14Coul : 1.f, vs = mm.z ? kScale14Vdw : 1.f;
-    float4 qa = R.xyzq[R.mov_atom[mm.x]], qb = R.xyzq[R.mov_atom[mm.y]];
-    float2 la = R.lj[R.mov_atom[mm.x]], lb = R.lj[R.mov_atom[mm.y]];
-    float r[3] = {s.mov[3 * mm.y] - s.mov[3 * mm.x], s.mov[3 * mm.y + 1] - s.mov[3 * mm.x + 1],
-                  s.mov[3 * mm.y + 2] - s.mov[3 * mm.x + 2]};
+    const int4 mm = R.mm[k];
+    const float4 pp = R.mm_par[k];
+    const float* pa = s.mov + 3 * mm.x;
+    const float* pb = s.mov + 3 * mm.y;
+    const float r[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
     float de;
-    e += pair_energy(r[0], r[1], r[2], qa.w * qb.w * cs, (la.x + lb.x) * 0.5f,
-                     la.y * lb.y * 4.f * vs, &de);
-    rec_grad_add(s, mov_ref(mm.x), r, -de);
-    rec_grad_add(s, mov_ref(mm.y), r, de);
+    e += pair_energy_qk(r[0], r[1], r[2], pp.x, pp.y, pp.z, &de);
+    const float* ja = s.jac + 3 * mm.x;
+    const float* jb = s.jac + 3 * mm.y;
+    atomicAdd(&s.torsg[mm.w & 0xffff], -de * dot3(ja, r));
+    atomicAdd(&s.torsg[mm.w >> 16], de * dot3(jb, r));
   }
   wave_sync();
   if (tab) return e;  // movable-vs-fixed pairs are all inside the leaf tables
@@ -1315,11 +1318,17 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
     s.torsg[k] += (float)td;
   }
   wave_sync();
-  for (int m = lane; m < R.n_mov; m += kWave) {
-    float v = s.jac[3 * m] * s.movg[3 * m] + s.jac[3 * m + 1] * s.movg[3 * m + 1] +
-              s.jac[3 * m + 2] * s.movg[3 * m + 2];
-    atomicAdd(&s.torsg[R.mov_leaf[m]], v);
-  }
+  // the slot gradients (s.movg) hold the receptor bonded terms' shares only
+  const int nl = R.n_leaves;
+  const bool movg_used = R.tab ? (R.bond_seg[nl] < R.n_bonds || R.angle_seg[nl] < R.n_angles ||
+                                  R.dih_seg[nl] < R.n_dih)
+                               : (R.n_bonds > 0 || R.n_angles > 0 || R.n_dih > 0);
+  if (movg_used)
+    for (int m = lane; m < R.n_mov; m += kWave) {
+      float v = s.jac[3 * m] * s.movg[3 * m] + s.jac[3 * m + 1] * s.movg[3 * m + 1] +
+                s.jac[3 * m + 2] * s.movg[3 * m + 2];
+      atomicAdd(&s.torsg[R.mov_leaf[m]], v);
+    }
   wave_sync();
   return wave_sum(e);
 }

@@ -56,9 +56,13 @@ struct DevReceptor {
   const int* rp_seg;          // [M+1] offsets
   const float4* rp_pos;       // partner x, y, z, q * coul_scale
   const float2* rp_lj;        // partner sigma, sqrt(eps) * 4 * lj_scale
-  // movable-vs-movable pairs (x = slot a, y = slot b, z = is14)
+  // movable-vs-movable pairs (x = slot a, y = slot b, z = is14, w = leaf of
+  // a | leaf of b << 16) and their
+  // parameters (q_a q_b scale kCoulomb, (sigma_a + sigma_b) / 2,
+  // 4 sqrt(eps_a eps_b) scale, 0), formed in FP64 on the host
   int n_mm;
   const int4* mm;
+  const float4* mm_par;
   // pscore interaction sets (receptor side)
   int n_hydro, n_cation, n_anion, n_ring5, n_ring6;
   int n_don_c, n_don_n, n_acc_c, n_acc_n;
