@@ -577,13 +577,17 @@ def main():
     roof["algorithmic_bytes_per_launch"] = Bt
     roof["algorithmic_flops_per_launch"] = F
     # the bound that holds in practice: VALU issue.  SQ_INSTS_VALU per wave
-    # (profile summary) x one wave per pose x 4 cycles per wave64 instruction,
-    # over the SIMDs (4 per CU) at the 2.4 GHz peak engine clock
+    # (profile summary) x one wave per pose x 4 cycles per wave64 instruction
+    # over the SIMDs (4 per CU) at the 2.4 GHz peak engine clock, as a
+    # fraction of the measured launch time: ~1 means the VALU instructions
+    # alone fill the launch (the ratio can pass 1 by the few percent that
+    # the model's 4 cycles overstate some instructions)
     valu, _ = profiled_value("valu_insts_per_wave")
     if valu:
         props = torch.cuda.get_device_properties(torch.cuda.current_device())
         simds = 4 * props.multi_processor_count
-        roof["valu_issue_frac"] = valu * 4 * n_poses / (simds * 2.4e9 * t_s)
+        roof["valu_insts_per_wave"] = valu
+        roof["valu_cycles_over_launch"] = valu * 4 * n_poses / (simds * 2.4e9 * t_s)
 
     out = {
         "metric": "L-BFGS-converged poses/sec + score+grad evals/sec vs plain GPU and CPU",
