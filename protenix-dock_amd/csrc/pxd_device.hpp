@@ -324,7 +324,7 @@ struct PoseLds {
   float* torsg;  // [kMaxLeaves]
   float4* movp;  // [kMaxMovable] charge, sigma, sqrt(epsilon) of each movable slot
   float* grad;   // [kMaxAtoms*3], overlaid by stage while the bonded terms run
-  float4* stage; // [kStageSlots] bonded-term gradient vectors of one term chunk
+  float* stage;  // [kStageSlots][3] bonded-term gradient vectors of one term chunk
 };
 
 constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
@@ -332,7 +332,7 @@ constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
 #define PXD_POSE_LDS_FLOATS                                                   \
   (pxd::kMaxAtoms * 3 + pxd::kMaxAtoms * 4 + pxd::kMaxMovable * 13 +         \
    pxd::kMaxLeaves * 2 +                                                      \
-   (pxd::kMaxAtoms * 3 > pxd::kStageSlots * 4 ? pxd::kMaxAtoms * 3 : pxd::kStageSlots * 4))
+   (pxd::kMaxAtoms * 3 > pxd::kStageSlots * 3 ? pxd::kMaxAtoms * 3 : pxd::kStageSlots * 3))
 
 // Offsets (floats) of the per-pose LDS arrays for ligands of up to ma atoms,
 // nm movable receptor slots and nlv leaves; every array starts on a 16-byte
@@ -341,7 +341,9 @@ struct PoseDims {
   int q, lj, slot, mov, movg, jac, tors, torsg, movp, grad, total;
 };
 
-__host__ __device__ constexpr PoseDims pose_dims(int ma, int nm, int nlv) {
+// sw: floats per bonded-term stage slot, 3 (compact: the launch-sized LDS
+// of the LDS-bound kernels) or 4 (one 16-byte LDS access per slot).
+__host__ __device__ constexpr PoseDims pose_dims(int ma, int nm, int nlv, int sw = 3) {
   auto r4 = [](int x) { return (x + 3) & ~3; };
   PoseDims d{};
   d.q = r4(3 * ma);
@@ -354,7 +356,7 @@ __host__ __device__ constexpr PoseDims pose_dims(int ma, int nm, int nlv) {
   d.torsg = d.tors + r4(nlv);
   d.movp = d.torsg + r4(nlv);
   d.grad = d.movp + 4 * nm;
-  const int g = r4(3 * ma) > 4 * kStageSlots ? r4(3 * ma) : 4 * kStageSlots;
+  const int g = r4(3 * ma) > sw * kStageSlots ? r4(3 * ma) : sw * kStageSlots;
   d.total = d.grad + g;
   return d;
 }
@@ -372,7 +374,7 @@ __device__ inline PoseLds carve_pose_lds(float* base, const PoseDims& d) {
   s.torsg = base + d.torsg;
   s.movp = reinterpret_cast<float4*>(base + d.movp);
   s.grad = base + d.grad;
-  s.stage = reinterpret_cast<float4*>(s.grad);
+  s.stage = s.grad;
   return s;
 }
 
@@ -579,17 +581,29 @@ __device__ inline double receptor_intra(const DevReceptor& R, PoseLds& s) {
 // slots are numbered atom by atom, so lane then sums the contiguous slots of
 // atoms lane and lane + 64 into its registers g0 / g1 (fixed order: the sums
 // are deterministic).  gw: the chunk's gather word of this lane.
+// SW floats per slot (3, or 4 with one 16-byte LDS access per slot).
+template <int SW>
+__device__ inline float3 stage_get(const PoseLds& s, int slot) {
+  if (SW == 4) {
+    const float4 v = reinterpret_cast<const float4*>(s.stage)[slot];
+    return make_float3(v.x, v.y, v.z);
+  }
+  const float* v = s.stage + 3 * slot;
+  return make_float3(v[0], v[1], v[2]);
+}
+
+template <int SW>
 __device__ inline void stage_gather(const PoseLds& s, uint32_t gw, float* g0, float* g1) {
   wave_sync();
   const int o0 = gw & 255, c0 = (gw >> 8) & 255, o1 = (gw >> 16) & 255, c1 = gw >> 24;
   for (int k = 0; k < c0; ++k) {
-    float4 v = s.stage[o0 + k];
+    const float3 v = stage_get<SW>(s, o0 + k);
     g0[0] += v.x;
     g0[1] += v.y;
     g0[2] += v.z;
   }
   for (int k = 0; k < c1; ++k) {
-    float4 v = s.stage[o1 + k];
+    const float3 v = stage_get<SW>(s, o1 + k);
     g1[0] += v.x;
     g1[1] += v.y;
     g1[2] += v.z;
@@ -597,8 +611,17 @@ __device__ inline void stage_gather(const PoseLds& s, uint32_t gw, float* g0, fl
   wave_sync();
 }
 
+template <int SW>
 __device__ inline void stage_put(const PoseLds& s, uint32_t slots, int r, const float* v, float f) {
-  s.stage[(slots >> (8 * r)) & 255] = make_float4(v[0] * f, v[1] * f, v[2] * f, 0.f);
+  const int slot = (slots >> (8 * r)) & 255;
+  if (SW == 4) {
+    reinterpret_cast<float4*>(s.stage)[slot] = make_float4(v[0] * f, v[1] * f, v[2] * f, 0.f);
+  } else {
+    float* p = s.stage + 3 * slot;
+    p[0] = v[0] * f;
+    p[1] = v[1] * f;
+    p[2] = v[2] * f;
+  }
 }
 
 // intra_molecular_interactions::put_gradients on the ligand (interaction.cpp:170-196):
@@ -606,6 +629,7 @@ __device__ inline void stage_put(const PoseLds& s, uint32_t slots, int r, const 
 // are loaded before the current one is evaluated), their gradients gathered
 // into g0 / g1; non-bonded pairs by the list (atomics into s.grad, which is
 // cleared here) only for ligands without partner masks.
+template <int SW>
 __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, PoseLds& s,
                                       float* g0, float* g1) {
   double e = 0.0;
@@ -664,10 +688,10 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
       float dl = r - kr.y;
       e += 0.5f * kr.x * dl * dl;
       float de = kr.x * (1.f - kr.y * safe_rcp(r));
-      stage_put(s, sl, 0, d, -de);
-      stage_put(s, sl, 1, d, de);
+      stage_put<SW>(s, sl, 0, d, -de);
+      stage_put<SW>(s, sl, 1, d, de);
     }
-    stage_gather(s, gw, g0, g1);
+    stage_gather<SW>(s, gw, g0, g1);
   }
   for (int c = 0; c < nac; ++c, ++chunk, a += kWave) {
     const uint32_t gw = gat[kWave * chunk];
@@ -686,11 +710,11 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
       e += 0.5f * kt.x * dl * dl;
       float de = kt.x * dl;
       float fj[3] = {fi[0] + fk[0], fi[1] + fk[1], fi[2] + fk[2]};
-      stage_put(s, sl, 0, fi, de);
-      stage_put(s, sl, 1, fj, -de);
-      stage_put(s, sl, 2, fk, de);
+      stage_put<SW>(s, sl, 0, fi, de);
+      stage_put<SW>(s, sl, 1, fj, -de);
+      stage_put<SW>(s, sl, 2, fk, de);
     }
-    stage_gather(s, gw, g0, g1);
+    stage_gather<SW>(s, gw, g0, g1);
   }
   for (int c = 0; c < ndc; ++c, ++chunk, t += kWave) {
     const uint32_t gw = gat[kWave * chunk];
@@ -711,12 +735,12 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
                                g[0], g[1], g[2], g[3]);
       float dedt;
       e += fourier(per, kp, th, &dedt);
-      stage_put(s, sl, 0, g[0], -dedt);
-      stage_put(s, sl, 1, g[1], -dedt);
-      stage_put(s, sl, 2, g[2], -dedt);
-      stage_put(s, sl, 3, g[3], -dedt);
+      stage_put<SW>(s, sl, 0, g[0], -dedt);
+      stage_put<SW>(s, sl, 1, g[1], -dedt);
+      stage_put<SW>(s, sl, 2, g[2], -dedt);
+      stage_put<SW>(s, sl, 3, g[3], -dedt);
     }
-    stage_gather(s, gw, g0, g1);
+    stage_gather<SW>(s, gw, g0, g1);
   }
   // pairs by partner masks (ligand_pairs_ring / _ring2): the gradient
   // stays in registers and is stored at the end of inter, so the region the
@@ -755,9 +779,10 @@ __device__ inline double ligand_intra(const DevLigands& L, const LigInfo& li, Po
 // a + k's sum in step k and moves one lane down the ring (ds_bpermute) after
 // each step; at the end it is sent home.  The partner bits come from the
 // owned atom's masks rotated so that bit k is partner a + k.  Partner data
-// come from 32-byte records (x, y, z, q, sigma / 2, sqrt(eps)) written twice,
+// come from 24-byte records (x, y, z, q, sigma / 2, sqrt(eps)) written twice,
 // at j and j + n, into the gradient region (free here: this mode keeps the
 // gradient in registers), so partner a + k is one pointer step away.
+template <int SW>
 __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& li,
                                            const PoseLds& s, float* g0) {
   const int lane = lane_id(), n = li.n_atoms;
@@ -782,14 +807,26 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   }
   const float x0 = s.xyz[3 * a], y0 = s.xyz[3 * a + 1], z0 = s.xyz[3 * a + 2], q0 = s.q[a];
   const float2 l0 = s.lj[a];
-  static_assert(2 * kWave * 8 <= 4 * kStageSlots, "2n records of 8 floats fit the gradient region");
-  float4* rec = reinterpret_cast<float4*>(s.grad);  // 2n records of 2 float4 (n <= 64)
+  // 2n records of float2 (x, y), (z, q), (sigma / 2, sqrt eps) (and, with
+  // 16-byte stage slots, a pad pair: 32-byte records), n <= 64
+  constexpr int RW = SW == 4 ? 4 : 3;  // float2 per record
+  static_assert(2 * kWave * 2 * RW <= SW * kStageSlots, "2n records fit the gradient region");
+  float2* rec = reinterpret_cast<float2*>(s.grad);
   if (act && !half) {
-    const float4 r0 = make_float4(x0, y0, z0, q0), r1 = make_float4(0.5f * l0.x, l0.y, 0.f, 0.f);
-    rec[2 * a] = r0;
-    rec[2 * a + 1] = r1;
-    rec[2 * (a + n)] = r0;
-    rec[2 * (a + n) + 1] = r1;
+    const float2 r0 = make_float2(x0, y0), r1 = make_float2(z0, q0),
+                 r2 = make_float2(0.5f * l0.x, l0.y);
+    if (RW == 4) {  // 32-byte records: one 16-byte access for (x, y, z, q)
+      const float4 r01 = make_float4(x0, y0, z0, q0);
+      *reinterpret_cast<float4*>(rec + RW * a) = r01;
+      *reinterpret_cast<float4*>(rec + RW * (a + n)) = r01;
+    } else {
+      rec[RW * a] = r0;
+      rec[RW * a + 1] = r1;
+      rec[RW * (a + n)] = r0;
+      rec[RW * (a + n) + 1] = r1;
+    }
+    rec[RW * a + 2] = r2;
+    rec[RW * (a + n) + 2] = r2;
   }
   wave_sync();
   // own factors of the pair parameters: q_a * kCoulomb, sigma_a / 2, 4 sqrt(eps_a)
@@ -800,7 +837,7 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
   const int K0 = split ? (K + 1) / 2 : K;  // steps per half: k0 + 1 .. k0 + K0
   const int k0 = half * K0;
   float gr[3] = {0.f, 0.f, 0.f}, t[3] = {0.f, 0.f, 0.f}, es = 0.f;
-  const float4* pr = rec + 2 * (a + 1 + k0);  // partner a + k's record in step k
+  const float2* pr = rec + RW * (a + 1 + k0);  // partner a + k's record in step k
   // bits 1 .. 32 of the rotated masks (K <= 32 steps), bit k - 1 = step k;
   // with n even, the pair (a, a + n/2) is met from a < n/2 only: lanes
   // a >= n/2 drop step n/2 (inactive lanes have empty masks).  The upper
@@ -815,16 +852,24 @@ __device__ inline double ligand_pairs_ring(const DevLigands& L, const LigInfo& l
     ba = (ba >> k0) & keep;
     bf = (bf >> k0) & keep;
   }
-  for (int k = 1; k <= K0; ++k, pr += 2) {
+  for (int k = 1; k <= K0; ++k, pr += RW) {
     const uint32_t ia = __builtin_amdgcn_ubfe(ba, k - 1, 1), i4 = __builtin_amdgcn_ubfe(bf, k - 1, 1);
     if (ia | i4) {
       const float fa = (float)ia, f4 = (float)i4;  // scales: 1, the 1-4 scale, or their sum
       const float cs = fmaf(f4, kScale14Coul, fa), vs = fmaf(f4, kScale14Vdw, fa);
-      const float4 pb = pr[0];
-      const float2 lb = *reinterpret_cast<const float2*>(pr + 1);
-      const float rx = pb.x - x0, ry = pb.y - y0, rz = pb.z - z0;
+      float2 pxy, pzq;
+      if (RW == 4) {
+        const float4 p4 = *reinterpret_cast<const float4*>(pr);
+        pxy = make_float2(p4.x, p4.y);
+        pzq = make_float2(p4.z, p4.w);
+      } else {
+        pxy = pr[0];
+        pzq = pr[1];
+      }
+      const float2 lb = pr[2];
+      const float rx = pxy.x - x0, ry = pxy.y - y0, rz = pzq.x - z0;
       float de;
-      es += pair_energy_qk(rx, ry, rz, q0k * pb.w * cs, sh0 + lb.x, e40 * lb.y * vs, &de);
+      es += pair_energy_qk(rx, ry, rz, q0k * pzq.y * cs, sh0 + lb.x, e40 * lb.y * vs, &de);
       gr[0] -= rx * de;
       gr[1] -= ry * de;
       gr[2] -= rz * de;
@@ -1290,6 +1335,7 @@ __device__ inline void leaf_lookup(const DevReceptor& R, int k, float theta, dou
 // the pose whose ligand coordinates and receptor torsions are in s.xyz /
 // s.tors.  Gradients are written to s.grad / s.torsg (overwrite).  Returns
 // the wave-uniform FP64 energy.  One wave per pose (wave_sync: no block barrier).
+template <int SW = 3>
 __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                                    const LigInfo& li, PoseLds& s) {
   const int lane = lane_id();
@@ -1300,10 +1346,10 @@ __device__ inline double eval_pose(const DevReceptor& R, const DevLigands& L, co
   if (lane == 0) e += R.e_const;
   wave_sync();
   float g0[3] = {0.f, 0.f, 0.f}, g1[3] = {0.f, 0.f, 0.f};
-  e += ligand_intra(L, li, s, g0, g1);
+  e += ligand_intra<SW>(L, li, s, g0, g1);
   if (li.pmode) {
     if (li.n_atoms <= kWave)
-      e += ligand_pairs_ring(L, li, s, g0);
+      e += ligand_pairs_ring<SW>(L, li, s, g0);
     else
       e += ligand_pairs_ring2(L, li, s, g0, g1);
   }

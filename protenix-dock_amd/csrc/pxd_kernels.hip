@@ -358,7 +358,8 @@ void launch_score(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
 // and the new point's: never more than three at once).
 enum { V_X = 0, V_G, V_GP, V_D, V_W0, V_W1, V_W2, V_COUNT };
 
-template <int VPL>
+// SW: floats per bonded-term stage slot of the pose arrays (pose_dims).
+template <int VPL, int SW = 3>
 struct LbfgsCtx {
   const DevReceptor* R;
   const DevLigands* L;
@@ -388,8 +389,8 @@ struct LbfgsCtx {
 // rotation vector, ligand torsions; receptor torsions already in c.s.tors):
 // free_ligand::apply_parameters (data.cpp:592-641) of the input pose about
 // its centre.  cen receives the rotation centre.
-template <int VPL>
-__device__ inline void dof_pose(LbfgsCtx<VPL>& c, float* cen) {
+template <int VPL, int SW>
+__device__ inline void dof_pose(LbfgsCtx<VPL, SW>& c, float* cen) {
   const int lane = lane_id();
   const LigInfo& li = c.li;
   cen[0] = c.c0[0] + c.dv[0];
@@ -459,8 +460,8 @@ struct IPair2 {
 };
 
 // obj_func_closure::evaluate_direction (optimizer.cpp:155-167): f and g at x + t d.
-template <int VPL>
-__device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, float* g) {
+template <int VPL, int SW>
+__device__ inline double eval_dir(LbfgsCtx<VPL, SW>& c, double t, const float* d, float* g) {
   const int lane = lane_id();
   const float* x = c.vec(V_X);
   float tf = (float)t;
@@ -477,7 +478,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
     __syncthreads();
     float cen[3], w[3] = {c.dv[3], c.dv[4], c.dv[5]};
     dof_pose(c, cen);
-    double f = eval_pose(*c.R, *c.L, c.G, c.li, c.s);
+    double f = eval_pose<SW>(*c.R, *c.L, c.G, c.li, c.s);
     dof_grad(*c.L, c.li, c.s.xyz, c.s.grad, cen, c.tq, c.dv);
     float tau[3] = {c.dv[3], c.dv[4], c.dv[5]}, jw[3];
     so3_jlt(w, tau, jw);
@@ -502,7 +503,7 @@ __device__ inline double eval_dir(LbfgsCtx<VPL>& c, double t, const float* d, fl
       c.s.tors[e - c.n3] = xv;
   }
   __syncthreads();
-  double f = eval_pose(*c.R, *c.L, c.G, c.li, c.s);
+  double f = eval_pose<SW>(*c.R, *c.L, c.G, c.li, c.s);
 #pragma unroll
   for (int k = 0; k < VPL; ++k) if (k < c.nk) {
     int e = lane + kWave * k;
@@ -533,8 +534,8 @@ __device__ inline double cubic_interpolate(double x1, double f1, double g1, doub
 // strong_wolfe (optimizer.cpp:235-427).  Gradient vectors are named buffers
 // in the LDS pool; the reference's std::move of vectors becomes a swap of
 // buffer indices (uniform scalars), so no vector is copied in the loop.
-template <int VPL>
-__device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* t_io,
+template <int VPL, int SW>
+__device__ void strong_wolfe(LbfgsCtx<VPL, SW>& c, const LbfgsParamsDev& p, double* t_io,
                              const float* d, double* f_io, double gtd) {
   double t = *t_io, f = *f_io;
   int b_new = V_W0, b_prev = V_W1;
@@ -656,8 +657,9 @@ __device__ void strong_wolfe(LbfgsCtx<VPL>& c, const LbfgsParamsDev& p, double* 
 // x = c.vec(V_X) is the start (ligand xyz then receptor torsions, c.n3 / c.n
 // set, c.nevals = 0); on return x is the result.  hy / hs: this wave's history
 // scratch (history x 64*VPL floats each).
-template <int VPL>
-__device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy, float* hs,
+// s_al: the two-loop's alphas, history + 1 entries (one slot for masked steps).
+template <int VPL, int SW>
+__device__ void lbfgs_solve(LbfgsCtx<VPL, SW>& c, const LbfgsParamsDev& P, float* hy, float* hs,
                             double* hr, double* s_al, double* energy_out,
                             bool* converged_out, bool evaluate_only = false) {
   const int lane = lane_id();
@@ -775,8 +777,9 @@ __device__ void lbfgs_solve(LbfgsCtx<VPL>& c, const LbfgsParamsDev& P, float* hy
             for (int k = 0; k < VPL; ++k) acc += bs[j][k] * qr[k];
             double al = (double)wave_sum(acc) * br[j];
             al = h >= 0 ? al : 0.0;
-            s_al[h >= 0 ? h : kMaxHistory] = al;  // every lane stores the same value
             const float alf = (float)al;
+            // every lane stores the same value; a masked step writes slot m
+            s_al[h >= 0 ? h : m] = al;
 #pragma unroll
             for (int k = 0; k < VPL; ++k) qr[k] -= alf * by[j][k];
             fetch(h - kPre, bs[j], by[j], br[j]);
@@ -872,7 +875,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
                                               const DevGrid* __restrict__ G, LbfgsArgs A,
                                               int vpl_lo) {
   __shared__ float vpool[V_COUNT * kWave * VPL];
-  __shared__ double s_al[kMaxHistory + 1];  // + one slot for masked steps
   __shared__ __align__(16) float dofl[MODE ? PXD_DOF_LDS_FLOATS : 1];
   __shared__ int s_pose;
   const int lane = lane_id();
@@ -885,7 +887,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
   c.L = &L;
   c.G = G;
   // pose arrays in dynamic LDS, sized for the batch's largest ligand
-  c.s = carve_pose_lds(dyn_lds, pose_dims(A.max_atoms, R.n_mov, R.n_leaves));
+  const PoseDims pdims = pose_dims(A.max_atoms, R.n_mov, R.n_leaves);
+  c.s = carve_pose_lds(dyn_lds, pdims);
+  // the two-loop's alphas (history + 1 doubles: one slot for masked steps)
+  // after the pose arrays in the launch-sized LDS
+  double* s_al = reinterpret_cast<double*>(dyn_lds + pdims.total);
   c.v = vpool;
   c.mode = MODE;
   if (MODE) {
@@ -962,11 +968,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLbfgsWaves)
   }
 }
 
+// Dynamic LDS of a k_lbfgs launch: the pose arrays for the batch's largest
+// ligand, then the two-loop's alphas (history + 1 doubles).
+static size_t lbfgs_lds_bytes(const DevReceptor& R, const LbfgsArgs& c) {
+  return sizeof(float) * pose_dims(c.max_atoms, R.n_mov, R.n_leaves).total +
+         sizeof(double) * (c.prm.history + 1);
+}
+
 void launch_lbfgs_class(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                         const LbfgsArgs& a, int cls, int n_slots, hipStream_t st) {
   LbfgsArgs c = a;
   c.max_atoms = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
-  const size_t lds = sizeof(float) * pose_dims(c.max_atoms, R.n_mov, R.n_leaves).total;
+  const size_t lds = lbfgs_lds_bytes(R, c);
   static_assert(kLbfgsClasses == 3, "one template per class below");
   if (cls == 2)
     hipLaunchKernelGGL((k_lbfgs<kLbfgsClassVpl[2], 0>), dim3(n_slots), dim3(kWave), lds, st, R,
@@ -983,7 +996,7 @@ void launch_lbfgs(const DevReceptor& R, const DevLigands& L, const DevGrid* G,
                   const LbfgsArgs& a, int n_slots, hipStream_t st) {
   LbfgsArgs c = a;
   c.max_atoms = a.max_atoms > 0 && a.max_atoms <= kMaxAtoms ? a.max_atoms : kMaxAtoms;
-  const size_t lds = sizeof(float) * pose_dims(c.max_atoms, R.n_mov, R.n_leaves).total;
+  const size_t lds = lbfgs_lds_bytes(R, c);
   // rigid-body + torsion variables: n <= 6 + 31 + 64 always fits width 2
   if (a.prm.dof_mode) {
     hipLaunchKernelGGL((k_lbfgs<2, 1>), dim3(n_slots), dim3(kWave), lds, st, R, L, G, c, 0);
@@ -1130,7 +1143,7 @@ struct McDims {
 __host__ __device__ constexpr McDims mc_dims(int ma, int mf, int nm, int nlv, int n_rings) {
   auto r4 = [](int x) { return (x + 3) & ~3; };
   McDims d{};
-  d.ring = pose_dims(ma, nm, nlv).total;
+  d.ring = pose_dims(ma, nm, nlv, 4).total;
   d.apply = d.ring + r4(6 * (n_rings + kMaxLigRings));
   d.cur = d.apply + r4(apply_lds_floats(ma, mf));
   d.cur_t = d.cur + r4(3 * ma);
@@ -1154,12 +1167,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kMcWaves))) 
   float* hy = A.hist + (size_t)blockIdx.x * lbfgs_slot_floats(A.hist_m);
   float* hs = hy + (size_t)A.hist_m * NV;
   double* hr = reinterpret_cast<double*>(hs + (size_t)A.hist_m * NV);
-  LbfgsCtx<VPL> c;
+  LbfgsCtx<VPL, 4> c;  // 16-byte stage slots: one LDS access each (registers, not LDS, bound this kernel)
   c.R = &R;
   c.L = &L;
   c.G = G;
   const McDims md = mc_dims(A.max_atoms, A.max_frags, R.n_mov, nl, R.n_ring5 + R.n_ring6);
-  c.s = carve_pose_lds(dyn_lds, pose_dims(A.max_atoms, R.n_mov, nl));
+  c.s = carve_pose_lds(dyn_lds, pose_dims(A.max_atoms, R.n_mov, nl, 4));
   c.v = vpool;
   c.mode = 0;
   PscoreLds ps;
