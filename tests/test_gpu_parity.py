@@ -334,6 +334,28 @@ def test_gpu_leaf_tables_match_direct_and_oracle(world):
         np.testing.assert_allclose(t1[p], t0[p], rtol=1e-4, atol=1e-3)
 
 
+def test_gpu_leaf_tables_huge_torsions_stay_in_range(world):
+    """Receptor torsions far outside any revolution (|theta| up to 3e38, the
+    FP32 range, and either side of the 2^20 rad switch to the exact fmod
+    reduction) and non-finite ones: the leaf-table index is reduced and
+    clamped into the table, so finite angles give finite energies and
+    gradients, and NaN / infinite angles give a NaN energy, without a fault."""
+    pose_lig, xyz, tors = _poses(world, 1, 77, spread=2.0)
+    n = len(pose_lig)
+    big = np.array([1e17, -1e17, 1e30, -1e30, 3e38, -3e38, 2.0 ** 21 + 0.5, -(2.0 ** 20) - 3.0])
+    tors = np.tile(big, (n, 1))[:, :8]
+    B = lib.LigandBatch(world["ligs"])
+    P = lib.PoseSet(B, pose_lig, xyz, 8, tors=tors)
+    e, g, t = [a.cpu().numpy() for a in lib.score_grad(world["R"], world["G"], P)]
+    assert np.isfinite(e).all() and np.isfinite(g).all() and np.isfinite(t).all()
+    bad = tors.copy()
+    bad[:, 0] = np.nan
+    bad[:, 1] = np.inf
+    Pb = lib.PoseSet(B, pose_lig, xyz, 8, tors=bad)
+    eb = lib.score_grad(world["R"], world["G"], Pb)[0].cpu().numpy()
+    assert np.isnan(eb).all()
+
+
 # ---------------------------------------------------------------- rigid-body + torsion variables
 
 def test_gpu_dof_grad_matches_oracle(world):
