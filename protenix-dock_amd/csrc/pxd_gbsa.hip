@@ -7,9 +7,13 @@
 // the receptor's).  All three are contiguous atom ranges of one per-pose
 // "complex" array, so each pass is one launch over (atom block, system,
 // pose):
+//   k_gb_fixed  once per call: for every fixed receptor atom, its
+//               descreening sum over the other fixed atoms (no pose moves
+//               them);
 //   k_gb_setup  complex coordinates and per-atom parameters of every pose;
-//   k_gb_born   the descreening sums I_j (an LDS-tiled all-pairs pass) and
-//               the OBC Born radii;
+//   k_gb_born   the descreening sums I_j (LDS-tiled passes: the sources that
+//               move with the pose, then the fixed ones or their
+//               precomputed sum) and the OBC Born radii;
 //   k_gb_egb    the generalized-Born row sums over j (all pairs again) plus
 //               each atom's surface term;
 //   k_gb_sum    the rows of each system summed in a fixed order.
@@ -40,6 +44,8 @@ struct GbArgs {
   int n_chunk;   // poses in this launch
   int pose0;     // first pose of the launch
   int n_rec;
+  int n_fix;     // receptor atoms [0, n_fix) are fixed, [n_fix, n_rec) movable
+  const double* sfix;  // [n_fix] descreening sum of each fixed atom over the other fixed atoms
   int stride;    // atoms per pose slot: largest ligand + receptor
   const int* pose_lig;
   const int* pose_off;
@@ -121,38 +127,107 @@ __device__ inline double gb_descreen(double r12, double sr, double orj) {
   return 0.5 * D;
 }
 
-__global__ __launch_bounds__(kGbBlock) void k_gb_born(GbArgs A) {
+// Descreening sum of each fixed receptor atom j over the other fixed atoms
+// (scorer.cpp:810-829 restricted to sources that never move): one thread per
+// target, the sources staged through LDS.  Launched once per call.
+__global__ __launch_bounds__(kGbBlock) void k_gb_fixed(const double4* __restrict__ rec_x,
+                                                       const double2* __restrict__ rec_p,
+                                                       int n_fix, double* __restrict__ sfix) {
   __shared__ double4 t_x[kGbBlock];
   __shared__ double t_sr[kGbBlock];
-  const int p = blockIdx.z, s = blockIdx.y;
-  const LigInfo li = A.info[A.pose_lig[A.pose0 + p]];
-  int a0, a1;
-  gb_range(s, li.n_atoms, A.n_rec, &a0, &a1);
-  if (a0 + (int)blockIdx.x * kGbBlock >= a1) return;  // the whole block is past the range
-  const int j = a0 + blockIdx.x * kGbBlock + threadIdx.x;
-  const bool act = j < a1;
-  const double4* X = A.cx + (size_t)p * A.stride;
-  const double4* P = A.cp + (size_t)p * A.stride;
-  const double4 xj = act ? X[j] : make_double4(0.0, 0.0, 0.0, 0.0);
-  const double4 pj = act ? P[j] : make_double4(1.0, 1.0, 1.0, 0.0);
+  const int j = blockIdx.x * kGbBlock + threadIdx.x;
+  const bool act = j < n_fix;
+  const double4 xj = act ? rec_x[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const double orj = act ? rec_p[j].x - PXD_GB_OFFSET : 1.0;
   double sum = 0.0;
-  for (int base = a0; base < a1; base += kGbBlock) {
+  for (int base = 0; base < n_fix; base += kGbBlock) {
     __syncthreads();
     const int i = base + threadIdx.x;
-    if (i < a1) {
-      t_x[threadIdx.x] = X[i];
-      t_sr[threadIdx.x] = P[i].y;
+    if (i < n_fix) {
+      t_x[threadIdx.x] = rec_x[i];
+      const double2 pi = rec_p[i];
+      t_sr[threadIdx.x] = (pi.x - PXD_GB_OFFSET) * pi.y;
     }
     __syncthreads();
-    const int cnt = min(kGbBlock, a1 - base);
+    const int cnt = min(kGbBlock, n_fix - base);
     if (act) {
       for (int u = 0; u < cnt; ++u) {
         if (base + u == j) continue;
         const double4 xi = t_x[u];
         const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        sum += gb_descreen(sqrt(dx * dx + dy * dy + dz * dz), t_sr[u], pj.x);
+        sum += gb_descreen(sqrt(dx * dx + dy * dy + dz * dz), t_sr[u], orj);
       }
     }
+  }
+  if (act) sfix[j] = sum;
+}
+
+// Accumulates into *sum the descreening of target j (position xj, offset
+// radius orj) by the sources src(v), v in [0, nv), staged through LDS in
+// tiles; every thread of the block calls it with the same nv.
+template <class Src>
+__device__ inline void gb_descreen_tiles(const double4* X, const double4* P, int nv, Src src,
+                                         bool act, int j, double4 xj, double orj, double* sum,
+                                         double4* t_x, double* t_sr, int* t_i) {
+  for (int base = 0; base < nv; base += kGbBlock) {
+    __syncthreads();
+    const int v = base + threadIdx.x;
+    if (v < nv) {
+      const int i = src(v);
+      t_x[threadIdx.x] = X[i];
+      t_sr[threadIdx.x] = P[i].y;
+      t_i[threadIdx.x] = i;
+    }
+    __syncthreads();
+    const int cnt = min(kGbBlock, nv - base);
+    if (act) {
+      for (int u = 0; u < cnt; ++u) {
+        if (t_i[u] == j) continue;
+        const double4 xi = t_x[u];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        *sum += gb_descreen(sqrt(dx * dx + dy * dy + dz * dz), t_sr[u], orj);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kGbBlock) void k_gb_born(GbArgs A) {
+  __shared__ double4 t_x[kGbBlock];
+  __shared__ double t_sr[kGbBlock];
+  __shared__ int t_i[kGbBlock];
+  const int p = blockIdx.z, s = blockIdx.y;
+  const LigInfo li = A.info[A.pose_lig[A.pose0 + p]];
+  const int n = li.n_atoms;
+  int a0, a1;
+  gb_range(s, n, A.n_rec, &a0, &a1);
+  const int b0 = a0 + (int)blockIdx.x * kGbBlock;
+  if (b0 >= a1) return;  // the whole block is past the range
+  const int j = b0 + threadIdx.x;
+  const bool act = j < a1;
+  const double4* X = A.cx + (size_t)p * A.stride;
+  const double4* P = A.cp + (size_t)p * A.stride;
+  const double4 xj = act ? X[j] : make_double4(0.0, 0.0, 0.0, 0.0);
+  const double4 pj = act ? P[j] : make_double4(1.0, 1.0, 1.0, 0.0);
+  // the system's sources that move with the pose: its ligand atoms (ligand
+  // and complex systems) and the movable receptor atoms [n + n_fix, n + n_rec)
+  const int nlig = s == 0 ? 0 : n;
+  const int nmov = s == 1 ? 0 : A.n_rec - A.n_fix;
+  const int nfix = s == 1 ? 0 : A.n_fix;
+  const int mov0 = n + A.n_fix;
+  double sum = 0.0;
+  gb_descreen_tiles(X, P, nlig + nmov, [&](int v) { return v < nlig ? v : mov0 + (v - nlig); },
+                    act, j, xj, pj.x, &sum, t_x, t_sr, t_i);
+  // the fixed receptor atoms [n, n + n_fix): a fixed target takes its
+  // precomputed sum, the others (ligand and movable atoms) sum them here; a
+  // block of fixed targets only skips the pass
+  const bool fixed_target = act && j >= n && j < mov0;
+  if (nfix > 0) {
+    const int bend = min(b0 + kGbBlock, a1);
+    const bool all_fixed = b0 >= n && bend <= mov0;
+    if (!all_fixed)
+      gb_descreen_tiles(X, P, nfix, [&](int v) { return n + v; }, act && !fixed_target, j, xj,
+                        pj.x, &sum, t_x, t_sr, t_i);
+    if (fixed_target) sum += A.sfix[j - n];
   }
   if (!act) return;
   // Born radius (scorer.cpp:832-839): psi = I * (radius - offset)
@@ -334,15 +409,26 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
     return gb_fail(PXD_ERR_HIP, "hipMemsetAsync", e);
   if (gb) {
     const int N = r->n_atoms, nl = r->n_leaves;
+    // receptor atoms in the device arrays: the fixed ones first (input
+    // order), then the movable ones; every sum over them is order-fixed
+    std::vector<int> leaf_of(N, -1), ord;
+    ord.reserve(N);
+    for (size_t m = 0; m < r->h_mov_atom.size(); ++m) leaf_of[r->h_mov_atom[m]] = r->h_mov_leaf[m];
+    for (int a = 0; a < N; ++a)
+      if (leaf_of[a] < 0) ord.push_back(a);
+    const int n_fix = (int)ord.size();
+    for (int a = 0; a < N; ++a)
+      if (leaf_of[a] >= 0) ord.push_back(a);
     std::vector<double4> rx(N);
     std::vector<double2> rp(N);
     std::vector<int> slot(N, -1);
-    for (int a = 0; a < N; ++a) {
-      rx[a] = make_double4(r->h_xyz64[3 * a], r->h_xyz64[3 * a + 1], r->h_xyz64[3 * a + 2],
+    for (int k = 0; k < N; ++k) {
+      const int a = ord[k];
+      rx[k] = make_double4(r->h_xyz64[3 * a], r->h_xyz64[3 * a + 1], r->h_xyz64[3 * a + 2],
                            r->h_q64[a]);
-      rp[a] = make_double2(rec_radius[a], rec_screen[a]);
+      rp[k] = make_double2(rec_radius[a], rec_screen[a]);
+      slot[k] = leaf_of[a];
     }
-    for (size_t m = 0; m < r->h_mov_atom.size(); ++m) slot[r->h_mov_atom[m]] = r->h_mov_leaf[m];
     std::vector<double4> lo(std::max(nl, 1)), la(std::max(nl, 1));
     for (int k = 0; k < nl; ++k) {
       lo[k] = make_double4(r->h_leaf_o[3 * k], r->h_leaf_o[3 * k + 1], r->h_leaf_o[3 * k + 2], 0.0);
@@ -354,7 +440,7 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
     double4 *d_rx, *d_lo, *d_la, *d_lp, *d_cx, *d_cp;
     double2* d_rp;
     int* d_slot;
-    double *d_born, *d_row;
+    double *d_born, *d_row, *d_sfix;
     const int stride = b->max_atoms + N;
     // poses per launch: about 192 MiB of per-pose workspace
     const size_t per_pose = (size_t)stride * (2 * sizeof(double4) + 6 * sizeof(double));
@@ -366,7 +452,8 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
         (e = buf.alloc(&d_cx, (size_t)chunk * stride)) != hipSuccess ||
         (e = buf.alloc(&d_cp, (size_t)chunk * stride)) != hipSuccess ||
         (e = buf.alloc(&d_born, (size_t)chunk * 3 * stride)) != hipSuccess ||
-        (e = buf.alloc(&d_row, (size_t)chunk * 3 * stride)) != hipSuccess)
+        (e = buf.alloc(&d_row, (size_t)chunk * 3 * stride)) != hipSuccess ||
+        (e = buf.alloc(&d_sfix, n_fix)) != hipSuccess)
       return gb_fail(PXD_ERR_HIP, "hipMalloc", e);
     if ((e = hipMemcpyAsync(d_rx, rx.data(), sizeof(double4) * N, hipMemcpyHostToDevice, st)) != hipSuccess ||
         (e = hipMemcpyAsync(d_rp, rp.data(), sizeof(double2) * N, hipMemcpyHostToDevice, st)) != hipSuccess ||
@@ -375,8 +462,13 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
         (e = hipMemcpyAsync(d_la, la.data(), sizeof(double4) * la.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
         (e = hipMemcpyAsync(d_lp, lp.data(), sizeof(double4) * lp.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
       return gb_fail(PXD_ERR_HIP, "hipMemcpyAsync", e);
+    if (n_fix > 0)
+      hipLaunchKernelGGL(k_gb_fixed, dim3((n_fix + kGbBlock - 1) / kGbBlock), dim3(kGbBlock), 0,
+                         st, d_rx, d_rp, n_fix, d_sfix);
     GbArgs A{};
     A.n_rec = N;
+    A.n_fix = n_fix;
+    A.sfix = d_sfix;
     A.stride = stride;
     A.pose_lig = pose_lig;
     A.pose_off = pose_atom_off;
