@@ -14,8 +14,8 @@
 //   k_gb_born   the descreening sums I_j (LDS-tiled passes: the sources that
 //               move with the pose, then the fixed ones or their
 //               precomputed sum) and the OBC Born radii;
-//   k_gb_egb    the generalized-Born row sums over j (all pairs again) plus
-//               each atom's surface term;
+//   k_gb_egb    the generalized-Born row sums (each unordered pair once,
+//               counted twice) plus each atom's surface term;
 //   k_gb_sum    the rows of each system summed in a fixed order.
 // Every sum has a fixed order, so the results are the same bits on every run.
 #include <hip/hip_runtime.h>
@@ -236,6 +236,9 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_born(GbArgs A) {
   A.born[((size_t)p * 3 + s) * A.stride + j] = 1.0 / (1.0 / pj.x - th / pj.z);
 }
 
+// The GB pair term is symmetric in i and j and only each system's total is
+// used, so row i sums the pairs (i, j > i) twice and its own diagonal term
+// once: each block starts at its own tile, half the ordered pairs overall.
 __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
   __shared__ double4 t_x[kGbBlock];
   __shared__ double t_b[kGbBlock];
@@ -243,8 +246,9 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
   const LigInfo li = A.info[A.pose_lig[A.pose0 + p]];
   int a0, a1;
   gb_range(s, li.n_atoms, A.n_rec, &a0, &a1);
-  if (a0 + (int)blockIdx.x * kGbBlock >= a1) return;
-  const int i = a0 + blockIdx.x * kGbBlock + threadIdx.x;
+  const int b0 = a0 + (int)blockIdx.x * kGbBlock;
+  if (b0 >= a1) return;
+  const int i = b0 + threadIdx.x;
   const bool act = i < a1;
   const double4* X = A.cx + (size_t)p * A.stride;
   const double* B = A.born + ((size_t)p * 3 + s) * A.stride;
@@ -252,7 +256,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
   const double bi = act ? B[i] : 1.0;
   const double pref = 0.5 * kGbCoulomb;  // 0.5 k (1 / solute - exp(-kappa f) / solvent)
   double egb = 0.0;
-  for (int base = a0; base < a1; base += kGbBlock) {
+  for (int base = b0; base < a1; base += kGbBlock) {
     __syncthreads();
     const int j = base + threadIdx.x;
     if (j < a1) {
@@ -263,19 +267,23 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
     const int cnt = min(kGbBlock, a1 - base);
     if (act) {
       for (int u = 0; u < cnt; ++u) {
+        const int jj = base + u;
+        if (jj < i) continue;
         const double4 xj = t_x[u];
-        double f;
-        if (base + u == i) {
+        double f, w;
+        if (jj == i) {
           f = bi;
+          w = 1.0;
         } else {
           const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
           const double r2 = dx * dx + dy * dy + dz * dz;
           const double bp = bi * t_b[u];
           f = sqrt(r2 + bp * exp(-0.25 * r2 / bp) + kGbEps);
+          w = 2.0;
         }
         const double L = pref * (1.0 / PXD_GB_SOLUTE_DIELECTRIC -
                                  exp(-PXD_GB_KAPPA * f) / PXD_GB_SOLVENT_DIELECTRIC);
-        egb -= xi.w * xj.w / f * L;
+        egb -= w * xi.w * xj.w / f * L;
       }
     }
   }
