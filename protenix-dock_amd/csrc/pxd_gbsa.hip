@@ -44,7 +44,7 @@ struct GbArgs {
   int n_chunk;   // poses in this launch
   int pose0;     // first pose of the launch
   int n_rec;
-  int n_fix;     // receptor atoms [0, n_fix) are fixed, [n_fix, n_rec) movable
+  int n_fix;     // receptor atoms [0, n_rec - n_fix) are movable, the rest fixed
   const double* sfix;  // [n_fix] descreening sum of each fixed atom over the other fixed atoms
   int stride;    // atoms per pose slot: largest ligand + receptor
   const int* pose_lig;
@@ -209,25 +209,24 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_born(GbArgs A) {
   const double4 xj = act ? X[j] : make_double4(0.0, 0.0, 0.0, 0.0);
   const double4 pj = act ? P[j] : make_double4(1.0, 1.0, 1.0, 0.0);
   // the system's sources that move with the pose: its ligand atoms (ligand
-  // and complex systems) and the movable receptor atoms [n + n_fix, n + n_rec)
-  const int nlig = s == 0 ? 0 : n;
-  const int nmov = s == 1 ? 0 : A.n_rec - A.n_fix;
+  // and complex systems) and the movable receptor atoms [n, fix0), which
+  // follow the ligand's
+  const int nmov_rec = A.n_rec - A.n_fix, fix0 = n + nmov_rec;
+  const int v0 = s == 0 ? n : 0, v1 = s == 1 ? n : fix0;
   const int nfix = s == 1 ? 0 : A.n_fix;
-  const int mov0 = n + A.n_fix;
   double sum = 0.0;
-  gb_descreen_tiles(X, P, nlig + nmov, [&](int v) { return v < nlig ? v : mov0 + (v - nlig); },
-                    act, j, xj, pj.x, &sum, t_x, t_sr, t_i);
-  // the fixed receptor atoms [n, n + n_fix): a fixed target takes its
-  // precomputed sum, the others (ligand and movable atoms) sum them here; a
-  // block of fixed targets only skips the pass
-  const bool fixed_target = act && j >= n && j < mov0;
+  gb_descreen_tiles(X, P, v1 - v0, [&](int v) { return v0 + v; }, act, j, xj, pj.x, &sum, t_x,
+                    t_sr, t_i);
+  // the fixed receptor atoms [fix0, n + n_rec): a fixed target takes its
+  // precomputed sum, the others (ligand and movable atoms, all in the
+  // system's first block) sum them here; a block of fixed targets skips the
+  // pass
+  const bool fixed_target = act && j >= fix0;
   if (nfix > 0) {
-    const int bend = min(b0 + kGbBlock, a1);
-    const bool all_fixed = b0 >= n && bend <= mov0;
-    if (!all_fixed)
-      gb_descreen_tiles(X, P, nfix, [&](int v) { return n + v; }, act && !fixed_target, j, xj,
+    if (b0 < fix0)
+      gb_descreen_tiles(X, P, nfix, [&](int v) { return fix0 + v; }, act && !fixed_target, j, xj,
                         pj.x, &sum, t_x, t_sr, t_i);
-    if (fixed_target) sum += A.sfix[j - n];
+    if (fixed_target) sum += A.sfix[j - fix0];
   }
   if (!act) return;
   // Born radius (scorer.cpp:832-839): psi = I * (radius - offset)
@@ -417,16 +416,17 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
     return gb_fail(PXD_ERR_HIP, "hipMemsetAsync", e);
   if (gb) {
     const int N = r->n_atoms, nl = r->n_leaves;
-    // receptor atoms in the device arrays: the fixed ones first (input
-    // order), then the movable ones; every sum over them is order-fixed
+    // receptor atoms in the device arrays: the movable ones first (input
+    // order), right after a pose's ligand atoms, then the fixed ones; every
+    // sum over them is order-fixed
     std::vector<int> leaf_of(N, -1), ord;
     ord.reserve(N);
     for (size_t m = 0; m < r->h_mov_atom.size(); ++m) leaf_of[r->h_mov_atom[m]] = r->h_mov_leaf[m];
     for (int a = 0; a < N; ++a)
-      if (leaf_of[a] < 0) ord.push_back(a);
-    const int n_fix = (int)ord.size();
-    for (int a = 0; a < N; ++a)
       if (leaf_of[a] >= 0) ord.push_back(a);
+    const int n_mov = (int)ord.size(), n_fix = N - n_mov;
+    for (int a = 0; a < N; ++a)
+      if (leaf_of[a] < 0) ord.push_back(a);
     std::vector<double4> rx(N);
     std::vector<double2> rp(N);
     std::vector<int> slot(N, -1);
@@ -472,7 +472,7 @@ pxd_status pxd_bscore_batch(const pxd_receptor* r, const double* rec_radius,
       return gb_fail(PXD_ERR_HIP, "hipMemcpyAsync", e);
     if (n_fix > 0)
       hipLaunchKernelGGL(k_gb_fixed, dim3((n_fix + kGbBlock - 1) / kGbBlock), dim3(kGbBlock), 0,
-                         st, d_rx, d_rp, n_fix, d_sfix);
+                         st, d_rx + n_mov, d_rp + n_mov, n_fix, d_sfix);
     GbArgs A{};
     A.n_rec = N;
     A.n_fix = n_fix;
