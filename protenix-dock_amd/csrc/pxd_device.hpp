@@ -324,7 +324,8 @@ struct PoseLds {
   float* torsg;  // [kMaxLeaves]
   float4* movp;  // [kMaxMovable] charge, sigma, sqrt(epsilon) of each movable slot
   float* grad;   // [kMaxAtoms*3], overlaid by stage while the bonded terms run
-  float* stage;  // [kStageSlots][3] bonded-term gradient vectors of one term chunk
+  float* stage;  // [kStageSlots][SW] bonded-term gradient vectors of one term chunk
+                 // (SW = 3 or 4 floats per slot: the evaluation's template parameter)
 };
 
 constexpr int kStageSlots = 4 * kWave;  // 64 dihedrals x 4 atoms
