@@ -241,6 +241,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_born(GbArgs A) {
 __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
   __shared__ double4 t_x[kGbBlock];
   __shared__ double t_b[kGbBlock];
+  __shared__ double t_ib[kGbBlock];  // 1 / (4 B_j): the exponent's divisor
   const int p = blockIdx.z, s = blockIdx.y;
   const LigInfo li = A.info[A.pose_lig[A.pose0 + p]];
   int a0, a1;
@@ -253,6 +254,7 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
   const double* B = A.born + ((size_t)p * 3 + s) * A.stride;
   const double4 xi = act ? X[i] : make_double4(0.0, 0.0, 0.0, 0.0);
   const double bi = act ? B[i] : 1.0;
+  const double ibi = 1.0 / bi;
   const double pref = 0.5 * kGbCoulomb;  // 0.5 k (1 / solute - exp(-kappa f) / solvent)
   double egb = 0.0;
   for (int base = b0; base < a1; base += kGbBlock) {
@@ -260,7 +262,9 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
     const int j = base + threadIdx.x;
     if (j < a1) {
       t_x[threadIdx.x] = X[j];
-      t_b[threadIdx.x] = B[j];
+      const double bj = B[j];
+      t_b[threadIdx.x] = bj;
+      t_ib[threadIdx.x] = 0.25 / bj;
     }
     __syncthreads();
     const int cnt = min(kGbBlock, a1 - base);
@@ -277,7 +281,8 @@ __global__ __launch_bounds__(kGbBlock) void k_gb_egb(GbArgs A) {
           const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
           const double r2 = dx * dx + dy * dy + dz * dz;
           const double bp = bi * t_b[u];
-          f = sqrt(r2 + bp * exp(-0.25 * r2 / bp) + kGbEps);
+          // r^2 / (4 B_i B_j) by the atoms' reciprocals: no division per pair
+          f = sqrt(r2 + bp * exp(-(r2 * ibi) * t_ib[u]) + kGbEps);
           w = 2.0;
         }
         const double L = pref * (1.0 / PXD_GB_SOLUTE_DIELECTRIC -
